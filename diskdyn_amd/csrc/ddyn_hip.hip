@@ -1,0 +1,1069 @@
+// ddyn_hip.hip -- MI355X (gfx950) kernels and C ABI for DiskDyn's dynamical hot path.
+//
+// One RK4 step runs, in this order (the reference order of src/grid.cu:387-389):
+//   for stage s = 0..3 of the massive set:
+//     k_pairs64 / k_pairs32   swarm-sourced pair forces on every massive body,
+//                             i-parallel over targets and split over source slices,
+//                             LDS-tiled sources, one FP64 partial per (slice, target)
+//     k_stage<s>              (stages 0-2) FP64 star/planet direct terms + slice
+//                             partials, then the RK4 stage update that writes the
+//                             next stage's positions (FP64 and an FP32 shadow)
+//   k_dust                    all four grain stages in registers (RK4calc_dust,
+//                             src/gpu_functions.cu:627-943) against the massive
+//                             stage positions; star terms in FP64, swarm tiles in LDS
+//   k_stage<3>                massive RK4 combination (RK4calc_massive_final,
+//                             src/gpu_functions.cu:613-625) fused into stage 4.
+//
+// The reference evaluates massive pairs on 32x32 tile pairs with Newton's third
+// law and CAS atomics (src/gpu_functions.cu:5-13, 101-217); here every target sums
+// its own sources in a fixed order, so results are deterministic and no atomics
+// are needed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ddyn.h"
+
+namespace {
+
+constexpr int kWG = 256;            // threads per workgroup (4 waves of 64)
+constexpr int kTile = 256;          // sources per LDS tile in the massive pair kernels
+constexpr int kDustTile32 = 1024;   // FP32 sources per LDS tile in the grain kernel (16 KiB)
+constexpr int kDustTile64 = 512;    // FP64 sources per LDS tile in the grain kernel (16 KiB)
+constexpr int kTargetWGs = 2048;    // pair-kernel workgroups wanted to fill 256 CUs
+constexpr double kLightAU = 63239.6717367;   // src/ddyn.cuh:22
+constexpr double kFar = 1.0e15;     // position of padding sources/targets [AU]
+constexpr float kTiny2 = 1.0e-20f;  // FP32 r^2 floor for unsoftened pairs (keeps self pairs finite)
+constexpr int kMaxBins = 2700;      // nsize, src/ddyn.cuh:35
+
+// ---------------------------------------------------------------------------
+// Massive swarm-pair kernels
+// ---------------------------------------------------------------------------
+// Targets: every massive body i in [0, n_grav).  Sources: swarm bodies j in the
+// slice [n_planets + y*slice_len, ...).  Lane l of wave w handles targets
+// wbase + l + 64k, k < TPT, so each wave covers one contiguous index range.
+// Pair law (src/gpu_functions.cu:157-172): a_i += mu_j (r_j - r_i) / R^3 with
+// R^2 = |r_j - r_i|^2 + eps^2 for swarm targets (eps = 0 for star/planet targets),
+// and a zero contribution when R^3 == 0.
+
+template <int TPT>
+__global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos, int n_grav,
+                                                 int n_planets, int slice_len, double eps2,
+                                                 double* __restrict__ part)
+{
+    __shared__ double4 tile[kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wbase = blockIdx.x * (kWG * TPT) + wave * (64 * TPT);
+    const int jbeg = n_planets + blockIdx.y * slice_len;
+    const int jend = min(jbeg + slice_len, n_grav);
+
+    double xi[TPT], yi[TPT], zi[TPT], e2[TPT], ax[TPT], ay[TPT], az[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int i = wbase + lane + 64 * k;
+        const double4 p = (i < n_grav) ? pos[i] : make_double4(kFar, kFar, kFar, 0.0);
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+        e2[k] = (i >= n_planets) ? eps2 : 0.0;
+        ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0;
+    }
+    for (int j0 = jbeg; j0 < jend; j0 += kTile) {
+        const int tl = min(kTile, jend - j0);
+        __syncthreads();
+        tile[threadIdx.x] = (threadIdx.x < tl) ? pos[j0 + threadIdx.x]
+                                               : make_double4(kFar, kFar, kFar, 0.0);
+        __syncthreads();
+#pragma unroll 2
+        for (int jj = 0; jj < kTile; ++jj) {
+            const double4 s = tile[jj];
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) + e2[k];
+                const double f = r2 * __builtin_sqrt(r2);
+                const double q = (f != 0.0) ? s.w / f : 0.0;
+                ax[k] = __builtin_fma(dx, q, ax[k]);
+                ay[k] = __builtin_fma(dy, q, ay[k]);
+                az[k] = __builtin_fma(dz, q, az[k]);
+            }
+        }
+    }
+    const size_t n = (size_t)n_grav;
+    double* px = part + (size_t)blockIdx.y * 3 * n;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int i = wbase + lane + 64 * k;
+        if (i < n_grav) { px[i] = ax[k]; px[n + i] = ay[k]; px[2 * n + i] = az[k]; }
+    }
+}
+
+// FP64 evaluation of one source tile for TPT targets (close pairs and the
+// diagonal tiles of the FP32 kernel).  Sources come from global memory.
+template <int TPT>
+__device__ inline void tile_fp64(const double4* __restrict__ pos, int j0, int j1,
+                                 const double* xi, const double* yi, const double* zi,
+                                 const double* e2, double* ax, double* ay, double* az)
+{
+    for (int j = j0; j < j1; ++j) {
+        const double4 s = pos[j];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+            const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) + e2[k];
+            const double f = r2 * __builtin_sqrt(r2);
+            const double q = (f != 0.0) ? s.w / f : 0.0;
+            ax[k] = __builtin_fma(dx, q, ax[k]);
+            ay[k] = __builtin_fma(dy, q, ay[k]);
+            az[k] = __builtin_fma(dz, q, az[k]);
+        }
+    }
+}
+
+// FP32-force form.  Each source tile is summed in FP32 (v_rsq_f32) into a
+// per-tile partial that is flushed into the FP64 accumulators.  A tile is
+// instead evaluated in FP64 for the whole wave when it holds one of the wave's
+// own targets (self pairs) or when any pair of the wave came closer than the
+// FP64 radius (r^2 < r64^2 + eps^2, tracked as a running minimum).
+template <int TPT>
+__global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos32,
+                                                 const double4* __restrict__ pos64, int n_grav,
+                                                 int n_planets, int slice_len, float eps2f,
+                                                 float r64sq, double eps2, double* __restrict__ part)
+{
+    __shared__ float4 tile[kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wbase = blockIdx.x * (kWG * TPT) + wave * (64 * TPT);
+    const int wend = wbase + 64 * TPT;
+    const int jbeg = n_planets + blockIdx.y * slice_len;
+    const int jend = min(jbeg + slice_len, n_grav);
+
+    float xf[TPT], yf[TPT], zf[TPT], e2f[TPT], thr[TPT];
+    double xi[TPT], yi[TPT], zi[TPT], e2d[TPT], ax[TPT], ay[TPT], az[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int i = wbase + lane + 64 * k;
+        const bool ok = i < n_grav;
+        const double4 p = ok ? pos64[i] : make_double4(kFar, kFar, kFar, 0.0);
+        const float4 q = ok ? pos32[i] : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+        xf[k] = q.x; yf[k] = q.y; zf[k] = q.z;
+        const bool sw = i >= n_planets;
+        e2d[k] = sw ? eps2 : 0.0;
+        e2f[k] = sw ? fmaxf(eps2f, kTiny2) : kTiny2;
+        thr[k] = r64sq + e2f[k];
+        ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0;
+    }
+    for (int j0 = jbeg; j0 < jend; j0 += kTile) {
+        const int tl = min(kTile, jend - j0);
+        __syncthreads();
+        tile[threadIdx.x] = (threadIdx.x < tl)
+                                ? pos32[j0 + threadIdx.x]
+                                : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
+        __syncthreads();
+        bool redo = (j0 < wend) && (j0 + tl > wbase);   // wave-uniform: own targets in tile
+        if (!redo) {
+            float fx[TPT], fy[TPT], fz[TPT], rmin[TPT];
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) { fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f; rmin[k] = 3.0e38f; }
+#pragma unroll 4
+            for (int jj = 0; jj < kTile; ++jj) {
+                const float4 s = tile[jj];
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    const float dx = s.x - xf[k], dy = s.y - yf[k], dz = s.z - zf[k];
+                    float r2 = __builtin_fmaf(dx, dx, e2f[k]);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                    rmin[k] = fminf(rmin[k], r2);
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float inv2 = inv * inv;
+                    const float q = (s.w * inv) * inv2;
+                    fx[k] = __builtin_fmaf(dx, q, fx[k]);
+                    fy[k] = __builtin_fmaf(dy, q, fy[k]);
+                    fz[k] = __builtin_fmaf(dz, q, fz[k]);
+                }
+            }
+            bool close = false;
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) close = close || (rmin[k] < thr[k]);
+            redo = __any(close);
+            if (!redo) {
+#pragma unroll
+                for (int k = 0; k < TPT; ++k) {
+                    ax[k] += (double)fx[k]; ay[k] += (double)fy[k]; az[k] += (double)fz[k];
+                }
+            }
+        }
+        if (redo) tile_fp64<TPT>(pos64, j0, j0 + tl, xi, yi, zi, e2d, ax, ay, az);
+    }
+    const size_t n = (size_t)n_grav;
+    double* px = part + (size_t)blockIdx.y * 3 * n;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int i = wbase + lane + 64 * k;
+        if (i < n_grav) { px[i] = ax[k]; px[n + i] = ay[k]; px[2 * n + i] = az[k]; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Massive stage epilogue: direct terms + slice partials + RK4 update
+// ---------------------------------------------------------------------------
+struct StageArgs {
+    int n_grav, n_planets, nslice;
+    const double* part;
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double* mu;
+    double4* p64[4];
+    float4* p32[4];
+    double *vrx, *vry, *vrz;     // stage velocity k{s}r
+    double *Srx, *Sry, *Srz;     // v + 2 v2 + 2 v3 + v4, built left to right
+    double *Sax, *Say, *Saz;     // a1 + 2 a2 + 2 a3 + a4, built left to right
+    double *ox, *oy, *oz;        // accel output (STAGE == 4)
+    double h;                    // dt/2, dt/2, dt for stages 0-2; dt/6 for stage 3
+};
+
+__device__ inline void store_pos(double4* p64, float4* p32, int i, double x, double y, double z,
+                                 double mu)
+{
+    p64[i] = make_double4(x, y, z, mu);
+    p32[i] = make_float4((float)x, (float)y, (float)z, (float)mu);
+}
+
+// STAGE 0..3: RK4 stage s; STAGE 4: write accelerations only.
+template <int STAGE>
+__global__ __launch_bounds__(kWG) void k_stage(StageArgs A)
+{
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= A.n_grav) return;
+    const double4* P = A.p64[STAGE == 4 ? 0 : STAGE];
+    const double4 pi = P[i];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    // Star and planets, FP64 and unsoftened (src/gpu_functions.cu:157-172).
+    for (int j = 0; j < A.n_planets; ++j) {
+        const double4 pj = P[j];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        const double f = r2 * __builtin_sqrt(r2);
+        const double q = (f != 0.0) ? pj.w / f : 0.0;
+        ax = __builtin_fma(dx, q, ax);
+        ay = __builtin_fma(dy, q, ay);
+        az = __builtin_fma(dz, q, az);
+    }
+    const size_t n = (size_t)A.n_grav;
+    for (int k = 0; k < A.nslice; ++k) {
+        const double* pk = A.part + (size_t)k * 3 * n;
+        ax += pk[i]; ay += pk[n + i]; az += pk[2 * n + i];
+    }
+    if (STAGE == 4) { A.ox[i] = ax; A.oy[i] = ay; A.oz[i] = az; return; }
+
+    const double x = A.x[i], y = A.y[i], z = A.z[i];
+    const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+    const double h = A.h;
+    if (STAGE == 0) {   // massive3 (src/gpu_functions.cu:219-230) with k1r = v
+        const double rx = __builtin_fma(ax, h, vx), ry = __builtin_fma(ay, h, vy),
+                     rz = __builtin_fma(az, h, vz);
+        store_pos(A.p64[1], A.p32[1], i, __builtin_fma(vx, h, x), __builtin_fma(vy, h, y),
+                  __builtin_fma(vz, h, z), pi.w);
+        A.vrx[i] = rx; A.vry[i] = ry; A.vrz[i] = rz;
+        A.Srx[i] = __builtin_fma(2.0, rx, vx);
+        A.Sry[i] = __builtin_fma(2.0, ry, vy);
+        A.Srz[i] = __builtin_fma(2.0, rz, vz);
+        A.Sax[i] = ax; A.Say[i] = ay; A.Saz[i] = az;
+    } else if (STAGE == 1 || STAGE == 2) {   // massive5 / massive7 (:350-362, :482-493)
+        const double ux = A.vrx[i], uy = A.vry[i], uz = A.vrz[i];
+        const double rx = __builtin_fma(ax, h, vx), ry = __builtin_fma(ay, h, vy),
+                     rz = __builtin_fma(az, h, vz);
+        store_pos(A.p64[STAGE + 1], A.p32[STAGE + 1], i, __builtin_fma(ux, h, x),
+                  __builtin_fma(uy, h, y), __builtin_fma(uz, h, z), pi.w);
+        if (STAGE == 1) {
+            A.vrx[i] = rx; A.vry[i] = ry; A.vrz[i] = rz;
+            A.Srx[i] = __builtin_fma(2.0, rx, A.Srx[i]);
+            A.Sry[i] = __builtin_fma(2.0, ry, A.Sry[i]);
+            A.Srz[i] = __builtin_fma(2.0, rz, A.Srz[i]);
+        } else {
+            A.Srx[i] = A.Srx[i] + rx; A.Sry[i] = A.Sry[i] + ry; A.Srz[i] = A.Srz[i] + rz;
+        }
+        A.Sax[i] = __builtin_fma(2.0, ax, A.Sax[i]);
+        A.Say[i] = __builtin_fma(2.0, ay, A.Say[i]);
+        A.Saz[i] = __builtin_fma(2.0, az, A.Saz[i]);
+    } else {   // STAGE 3: massive_final (:613-625), h = dt/6
+        const double nx = __builtin_fma(h, A.Srx[i], x), ny = __builtin_fma(h, A.Sry[i], y),
+                     nz = __builtin_fma(h, A.Srz[i], z);
+        A.x[i] = nx; A.y[i] = ny; A.z[i] = nz;
+        A.vx[i] = __builtin_fma(h, A.Sax[i] + ax, vx);
+        A.vy[i] = __builtin_fma(h, A.Say[i] + ay, vy);
+        A.vz[i] = __builtin_fma(h, A.Saz[i] + az, vz);
+        store_pos(A.p64[0], A.p32[0], i, nx, ny, nz, pi.w);
+    }
+}
+
+// Stage-0 positions from the base state (after upload).
+__global__ void k_init_pos(int n_grav, const double* x, const double* y, const double* z,
+                           const double* mu, double4* p64, float4* p32)
+{
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    if (i < n_grav) store_pos(p64, p32, i, x[i], y[i], z[i], mu[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Grain kernel: RK4calc_dust (src/gpu_functions.cu:627-943), one grain per lane
+// ---------------------------------------------------------------------------
+struct DustArgs {
+    int n_grav, n_planets, n_dust, n_bins;
+    const double4* p64[4];
+    const float4* p32[4];
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double* mu;
+    double* N;
+    const int32_t* id;
+    const double *beta_p, *betasw_p;
+    const float *beta_t, *betasw_t, *q_t;
+    double dt, eps2, Rmin2, vsw, Wz, Bstar, Rstar, light;
+    float eps2f, thr2f;
+    int SWQ, FLQ;
+    double *ox, *oy, *oz;   // accel mode outputs (n_dust)
+    int32_t* removed;       // accel mode outputs (n_dust)
+};
+
+// Star term with radiation pressure, PR and stellar-wind drag and the Lorentz
+// force (src/gpu_functions.cu:679-718).  Returns true if the grain sublimates.
+__device__ inline bool star_term(const DustArgs& A, const double4 s0, double px, double py,
+                                 double pz, double ux, double uy, double uz, double bet,
+                                 double betsw, double qe, double m, double& ax, double& ay,
+                                 double& az)
+{
+    const double dx = px - s0.x, dy = py - s0.y, dz = pz - s0.z;
+    const double RR = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    const double RR3 = RR * __builtin_sqrt(RR);
+    if (RR <= A.Rmin2) return true;
+    const double m0 = s0.w;
+    const double Fg = (1.0 - bet - betsw) * m0 / RR3;
+    const double Fb1 = (A.SWQ == 1) ? m0 / RR * (bet / A.light + betsw / A.vsw)
+                                    : bet * m0 / A.light / RR;
+    const double Fb2 = __builtin_fma(dz, uz, __builtin_fma(dy, uy, dx * ux)) / RR;
+    const double Fdx = Fb1 * __builtin_fma(Fb2, dx, ux);
+    const double Fdy = Fb1 * __builtin_fma(Fb2, dy, uy);
+    const double Fdz = Fb1 * __builtin_fma(Fb2, dz, uz);
+    double Flx = 0.0, Fly = 0.0;
+    if (A.FLQ == 1) {
+        const double Bz = A.Bstar * A.Rstar * A.Rstar * A.Rstar / RR3;
+        Flx = qe * __builtin_fma(-A.Wz, dx, uy) * Bz / m;
+        Fly = -qe * __builtin_fma(A.Wz, dy, ux) * Bz / m;
+    }
+    ax = __builtin_fma(-Fg, dx, -Fdx) + Flx;
+    ay = __builtin_fma(-Fg, dy, -Fdy) + Fly;
+    az = __builtin_fma(-Fg, dz, -Fdz);
+    return false;
+}
+
+// FP64 pull of sources [j0, j1) on the grain; `soft` adds eps^2.  Sets `hit`
+// on an exact coincidence (src/gpu_functions.cu:727-730).
+__device__ inline void grain_fp64(const double4* __restrict__ P, int j0, int j1, double px,
+                                  double py, double pz, double e2, double& ax, double& ay,
+                                  double& az, bool& hit)
+{
+    for (int j = j0; j < j1; ++j) {
+        const double4 s = P[j];
+        const double ex = px - s.x, ey = py - s.y, ez = pz - s.z;
+        const double RRu = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
+        hit = hit || (RRu == 0.0);
+        const double R2 = RRu + e2;
+        const double f = s.w / (R2 * __builtin_sqrt(R2));
+        ax = __builtin_fma(-f, ex, ax);
+        ay = __builtin_fma(-f, ey, ay);
+        az = __builtin_fma(-f, ez, az);
+    }
+}
+
+template <bool F32, bool ACCEL>
+__global__ __launch_bounds__(kWG) void k_dust(DustArgs A)
+{
+    __shared__ double4 lds64[kDustTile64];   // 16 KiB; the FP32 form views it as float4[1024]
+    float4* lds32 = reinterpret_cast<float4*>(lds64);
+    const int g = blockIdx.x * kWG + threadIdx.x;
+    const int64_t t = (int64_t)A.n_grav + g;
+    bool live = false;
+    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0, m = 1, bet = 0, betsw = 0, qe = 0;
+    if (g < A.n_dust) {
+        live = ACCEL || A.N[t] != 0.0;
+        x0 = A.x[t]; y0 = A.y[t]; z0 = A.z[t];
+        vx0 = A.vx[t]; vy0 = A.vy[t]; vz0 = A.vz[t];
+        m = A.mu[t];
+        const int32_t id = A.id[t];
+        if (A.beta_p) {
+            bet = A.beta_p[g];
+            betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+        } else if (id >= 0 && id < A.n_bins) {
+            bet = (double)A.beta_t[id];
+            betsw = (double)A.betasw_t[id];
+        }
+        qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
+    }
+    const double dt = A.dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
+    double px = x0, py = y0, pz = z0, ux = vx0, uy = vy0, uz = vz0;   // stage position, velocity
+    double kx[4], ky[4], kz[4], rx[4], ry[4], rz[4];
+    bool gone = false;
+    constexpr int kStages = ACCEL ? 1 : 4;
+#pragma unroll 1
+    for (int s = 0; s < kStages; ++s) {
+        const double4* P = A.p64[s];
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (live && !gone) {
+            gone = star_term(A, P[0], px, py, pz, ux, uy, uz, bet, betsw, qe, m, ax, ay, az);
+            bool hit = false;
+            grain_fp64(P, 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
+            gone = gone || hit;
+        }
+        // Swarm sources, LDS tiles (all threads take part in the barriers).
+        if (F32) {
+            const float xf = (float)px, yf = (float)py, zf = (float)pz;
+            const float e2f = fmaxf(A.eps2f, kTiny2);
+            for (int j0 = A.n_planets; j0 < A.n_grav; j0 += kDustTile32) {
+                const int tl = min(kDustTile32, A.n_grav - j0);
+                __syncthreads();
+                for (int q = threadIdx.x; q < kDustTile32; q += kWG)
+                    lds32[q] = (q < tl) ? A.p32[s][j0 + q]
+                                        : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
+                __syncthreads();
+                const bool act = live && !gone;
+                float fx = 0.f, fy = 0.f, fz = 0.f, rmin = 3.0e38f;
+#pragma unroll 4
+                for (int jj = 0; jj < kDustTile32; ++jj) {
+                    const float4 sj = lds32[jj];
+                    const float dx = sj.x - xf, dy = sj.y - yf, dz = sj.z - zf;
+                    float r2 = __builtin_fmaf(dx, dx, e2f);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                    rmin = fminf(rmin, r2);
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float inv2 = inv * inv;
+                    const float q = (sj.w * inv) * inv2;
+                    fx = __builtin_fmaf(dx, q, fx);
+                    fy = __builtin_fmaf(dy, q, fy);
+                    fz = __builtin_fmaf(dz, q, fz);
+                }
+                const bool redo = __any(act && rmin < A.thr2f);
+                if (act) {
+                    if (!redo) {
+                        ax += (double)fx; ay += (double)fy; az += (double)fz;
+                    } else {
+                        bool hit = false;
+                        grain_fp64(P, j0, j0 + tl, px, py, pz, A.eps2, ax, ay, az, hit);
+                    }
+                }
+            }
+        } else {
+            for (int j0 = A.n_planets; j0 < A.n_grav; j0 += kDustTile64) {
+                const int tl = min(kDustTile64, A.n_grav - j0);
+                __syncthreads();
+                for (int q = threadIdx.x; q < kDustTile64; q += kWG)
+                    lds64[q] = (q < tl) ? P[j0 + q] : make_double4(kFar, kFar, kFar, 0.0);
+                __syncthreads();
+                if (live && !gone) {
+                    bool hit = false;
+                    for (int jj = 0; jj < tl; ++jj) {
+                        const double4 sj = lds64[jj];
+                        const double ex = px - sj.x, ey = py - sj.y, ez = pz - sj.z;
+                        const double RRu = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
+                        hit = hit || (RRu == 0.0);
+                        const double R2 = RRu + A.eps2;
+                        const double f = sj.w / (R2 * __builtin_sqrt(R2));
+                        ax = __builtin_fma(-f, ex, ax);
+                        ay = __builtin_fma(-f, ey, ay);
+                        az = __builtin_fma(-f, ez, az);
+                    }
+                    gone = gone || hit;
+                }
+            }
+        }
+        kx[s] = ax; ky[s] = ay; kz[s] = az;
+        if (ACCEL) break;
+        // Next stage (src/gpu_functions.cu:737-743, 800-806, 863-869).
+        if (s == 0) {
+            rx[1] = __builtin_fma(ax, dt2, vx0); ry[1] = __builtin_fma(ay, dt2, vy0);
+            rz[1] = __builtin_fma(az, dt2, vz0);
+            px = __builtin_fma(vx0, dt2, x0); py = __builtin_fma(vy0, dt2, y0);
+            pz = __builtin_fma(vz0, dt2, z0);
+            ux = rx[1]; uy = ry[1]; uz = rz[1];
+        } else if (s == 1) {
+            rx[2] = __builtin_fma(ax, dt2, vx0); ry[2] = __builtin_fma(ay, dt2, vy0);
+            rz[2] = __builtin_fma(az, dt2, vz0);
+            px = __builtin_fma(rx[1], dt2, x0); py = __builtin_fma(ry[1], dt2, y0);
+            pz = __builtin_fma(rz[1], dt2, z0);
+            ux = rx[2]; uy = ry[2]; uz = rz[2];
+        } else if (s == 2) {
+            rx[3] = __builtin_fma(ax, dt, vx0); ry[3] = __builtin_fma(ay, dt, vy0);
+            rz[3] = __builtin_fma(az, dt, vz0);
+            px = __builtin_fma(rx[2], dt, x0); py = __builtin_fma(ry[2], dt, y0);
+            pz = __builtin_fma(rz[2], dt, z0);
+            ux = rx[3]; uy = ry[3]; uz = rz[3];
+        }
+    }
+    if (g >= A.n_dust) return;
+    if (ACCEL) {
+        A.ox[g] = kx[0]; A.oy[g] = ky[0]; A.oz[g] = kz[0];
+        A.removed[g] = gone ? 1 : 0;
+        return;
+    }
+    if (!live) return;
+    if (gone) { A.N[t] = 0.0; return; }
+    // Evolve (src/gpu_functions.cu:927-933).
+    A.x[t] = __builtin_fma(dt6, __builtin_fma(2.0, rx[1] + rx[2], vx0) + rx[3], x0);
+    A.y[t] = __builtin_fma(dt6, __builtin_fma(2.0, ry[1] + ry[2], vy0) + ry[3], y0);
+    A.z[t] = __builtin_fma(dt6, __builtin_fma(2.0, rz[1] + rz[2], vz0) + rz[3], z0);
+    A.vx[t] = __builtin_fma(dt6, __builtin_fma(2.0, kx[1] + kx[2], kx[0]) + kx[3], vx0);
+    A.vy[t] = __builtin_fma(dt6, __builtin_fma(2.0, ky[1] + ky[2], ky[0]) + ky[3], vy0);
+    A.vz[t] = __builtin_fma(dt6, __builtin_fma(2.0, kz[1] + kz[2], kz[0]) + kz[3], vz0);
+}
+
+// ---------------------------------------------------------------------------
+// Potential of the massive set (diagnostics): phi_i = -sum_{j != i} mu_j / R_ij
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_potential(const double4* __restrict__ pos, int n_grav,
+                                                   int n_planets, double eps2,
+                                                   double* __restrict__ phi)
+{
+    __shared__ double4 tile[kTile];
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    const double4 pi = (i < n_grav) ? pos[i] : make_double4(kFar, kFar, kFar, 0.0);
+    const bool sw = i >= n_planets;
+    double u = 0.0;
+    for (int j0 = 0; j0 < n_grav; j0 += kTile) {
+        const int tl = min(kTile, n_grav - j0);
+        __syncthreads();
+        if (threadIdx.x < tl) tile[threadIdx.x] = pos[j0 + threadIdx.x];
+        __syncthreads();
+        for (int jj = 0; jj < tl; ++jj) {
+            const int j = j0 + jj;
+            const double4 s = tile[jj];
+            const double dx = s.x - pi.x, dy = s.y - pi.y, dz = s.z - pi.z;
+            const double R2 = dx * dx + dy * dy + dz * dz + ((sw && j >= n_planets) ? eps2 : 0.0);
+            if (j != i && R2 > 0.0) u -= s.w / __builtin_sqrt(R2);
+        }
+    }
+    if (i < n_grav) phi[i] = u;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context and C ABI
+// ---------------------------------------------------------------------------
+struct ddyn_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    ddyn_params prm{};
+    bool have_params = false;
+    int64_t n_grav = 0, n_dust = 0;
+    // full state, n_all entries
+    double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *mu = nullptr, *N = nullptr;
+    int32_t* id = nullptr;
+    // massive stage buffers, n_grav entries
+    double4* p64[4] = {nullptr, nullptr, nullptr, nullptr};
+    float4* p32[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* stage = nullptr;     // 9 * n_grav: vr, Sr, Sa
+    double* part = nullptr;      // nslice * 3 * n_grav
+    int nslice = 0, slice_len = 0, tpt = 1;
+    double* acc = nullptr;       // 3 * n_all accel scratch
+    int32_t* rem = nullptr;      // n_dust accel scratch
+    // grain coefficients
+    double *beta_p = nullptr, *betasw_p = nullptr;
+    bool use_beta_p = false, have_betasw_p = false;
+    float *beta_t = nullptr, *betasw_t = nullptr, *q_t = nullptr;
+    int32_t n_bins = 0;
+    // timing
+    bool timing = false;
+    struct Ev { int cls; hipEvent_t a, b; };
+    std::vector<Ev> events;
+    int64_t t_launch[DDYN_KCLASS_COUNT] = {0, 0, 0};
+    double t_ms[DDYN_KCLASS_COUNT] = {0, 0, 0};
+};
+
+namespace {
+
+int fail(ddyn_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hipfail(ddyn_ctx* c, hipError_t e, const char* where)
+{
+    return fail(c, DDYN_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define DDYN_HIP(c, call)                                          \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return hipfail((c), e_, #call);      \
+    } while (0)
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_state(ddyn_ctx* c)
+{
+    dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
+    dfree(c->mu); dfree(c->N); dfree(c->id);
+    for (int s = 0; s < 4; ++s) { dfree(c->p64[s]); dfree(c->p32[s]); }
+    dfree(c->stage); dfree(c->part); dfree(c->acc); dfree(c->rem);
+    dfree(c->beta_p); dfree(c->betasw_p);
+    c->use_beta_p = false; c->have_betasw_p = false;
+    c->n_grav = c->n_dust = 0;
+    c->nslice = 0;
+}
+
+template <class T>
+int dalloc(ddyn_ctx* c, T*& p, size_t count)
+{
+    p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; return fail(c, DDYN_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+    return DDYN_OK;
+}
+
+struct TimedLaunch {
+    ddyn_ctx* c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(ddyn_ctx* c_, int cls_) : c(c_), cls(cls_)
+    {
+        if (!c->timing) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+        (void)hipEventRecord(a, c->stream);
+    }
+    ~TimedLaunch()
+    {
+        if (!a) return;
+        (void)hipEventRecord(b, c->stream);
+        c->events.push_back({cls, a, b});
+    }
+};
+
+void drain_events(ddyn_ctx* c)
+{
+    for (auto& e : c->events) {
+        float ms = 0.f;
+        if (hipEventSynchronize(e.b) == hipSuccess && hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            c->t_ms[e.cls] += ms;
+            c->t_launch[e.cls] += 1;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->events.clear();
+}
+
+int n_swarm(const ddyn_ctx* c) { return (int)(c->n_grav - c->prm.n_planets); }
+
+// Choose targets per lane and the source-slice split so that the pair kernel
+// launches about kTargetWGs workgroups.
+void plan_pairs(ddyn_ctx* c)
+{
+    const int ns = n_swarm(c);
+    c->tpt = 1;
+    if (ns <= 0) { c->nslice = 0; c->slice_len = 0; return; }
+    const int tblocks = (int)((c->n_grav + (int64_t)kWG * c->tpt - 1) / ((int64_t)kWG * c->tpt));
+    int want = (kTargetWGs + tblocks - 1) / tblocks;
+    const int max_slices = (ns + kTile - 1) / kTile;
+    if (want > max_slices) want = max_slices;
+    if (want < 1) want = 1;
+    int len = (ns + want - 1) / want;
+    len = ((len + kTile - 1) / kTile) * kTile;
+    c->slice_len = len;
+    c->nslice = (ns + len - 1) / len;
+}
+
+int check_ready(ddyn_ctx* c)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (!c->have_params) return fail(c, DDYN_ERR_STATE, "ddyn_set_params has not been called");
+    if (!c->x) return fail(c, DDYN_ERR_STATE, "ddyn_upload_state has not been called");
+    if (c->prm.n_planets > c->n_grav)
+        return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
+    return DDYN_OK;
+}
+
+double eps2_of(const ddyn_ctx* c) { return c->prm.eps * c->prm.eps; }
+
+double r64_of(const ddyn_ctx* c)
+{
+    return c->prm.fp64_radius > 0.0 ? c->prm.fp64_radius : 10.0 * c->prm.eps;
+}
+
+// Swarm-pair forces at stage s into c->part.
+int launch_pairs(ddyn_ctx* c, int s)
+{
+    if (c->nslice == 0) return DDYN_OK;
+    TimedLaunch tl(c, DDYN_KCLASS_PAIRS);
+    const int ng = (int)c->n_grav, np = c->prm.n_planets;
+    dim3 grid((unsigned)((ng + kWG * c->tpt - 1) / (kWG * c->tpt)), (unsigned)c->nslice);
+    if (c->prm.precision == DDYN_FP32_FORCE) {
+        const double r64 = r64_of(c);
+        k_pairs32<1><<<grid, kWG, 0, c->stream>>>(c->p32[s], c->p64[s], ng, np, c->slice_len,
+                                                   (float)eps2_of(c), (float)(r64 * r64),
+                                                   eps2_of(c), c->part);
+    } else {
+        k_pairs64<1><<<grid, kWG, 0, c->stream>>>(c->p64[s], ng, np, c->slice_len, eps2_of(c),
+                                                   c->part);
+    }
+    DDYN_HIP(c, hipGetLastError());
+    return DDYN_OK;
+}
+
+StageArgs stage_args(ddyn_ctx* c, double h)
+{
+    StageArgs A{};
+    const size_t n = (size_t)c->n_grav;
+    A.n_grav = (int)c->n_grav; A.n_planets = c->prm.n_planets; A.nslice = c->nslice;
+    A.part = c->part;
+    A.x = c->x; A.y = c->y; A.z = c->z; A.vx = c->vx; A.vy = c->vy; A.vz = c->vz; A.mu = c->mu;
+    for (int s = 0; s < 4; ++s) { A.p64[s] = c->p64[s]; A.p32[s] = c->p32[s]; }
+    A.vrx = c->stage; A.vry = c->stage + n; A.vrz = c->stage + 2 * n;
+    A.Srx = c->stage + 3 * n; A.Sry = c->stage + 4 * n; A.Srz = c->stage + 5 * n;
+    A.Sax = c->stage + 6 * n; A.Say = c->stage + 7 * n; A.Saz = c->stage + 8 * n;
+    A.h = h;
+    return A;
+}
+
+int launch_stage(ddyn_ctx* c, int s, const StageArgs& A)
+{
+    TimedLaunch tl(c, DDYN_KCLASS_STAGE);
+    const unsigned blocks = (unsigned)((c->n_grav + kWG - 1) / kWG);
+    switch (s) {
+        case 0: k_stage<0><<<blocks, kWG, 0, c->stream>>>(A); break;
+        case 1: k_stage<1><<<blocks, kWG, 0, c->stream>>>(A); break;
+        case 2: k_stage<2><<<blocks, kWG, 0, c->stream>>>(A); break;
+        case 3: k_stage<3><<<blocks, kWG, 0, c->stream>>>(A); break;
+        default: k_stage<4><<<blocks, kWG, 0, c->stream>>>(A); break;
+    }
+    DDYN_HIP(c, hipGetLastError());
+    return DDYN_OK;
+}
+
+DustArgs dust_args(ddyn_ctx* c, double dt)
+{
+    DustArgs A{};
+    A.n_grav = (int)c->n_grav; A.n_planets = c->prm.n_planets; A.n_dust = (int)c->n_dust;
+    A.n_bins = c->n_bins;
+    for (int s = 0; s < 4; ++s) { A.p64[s] = c->p64[s]; A.p32[s] = c->p32[s]; }
+    A.x = c->x; A.y = c->y; A.z = c->z; A.vx = c->vx; A.vy = c->vy; A.vz = c->vz;
+    A.mu = c->mu; A.N = c->N; A.id = c->id;
+    A.beta_p = c->use_beta_p ? c->beta_p : nullptr;
+    A.betasw_p = (c->use_beta_p && c->have_betasw_p) ? c->betasw_p : nullptr;
+    A.beta_t = c->beta_t; A.betasw_t = c->betasw_t; A.q_t = c->q_t;
+    A.dt = dt;
+    A.eps2 = eps2_of(c);
+    A.Rmin2 = c->prm.Rmin * c->prm.Rmin;
+    A.vsw = c->prm.vsw; A.Wz = c->prm.Wz; A.Bstar = c->prm.Bstar; A.Rstar = c->prm.Rstar;
+    A.light = c->prm.light > 0.0 ? c->prm.light : kLightAU;
+    A.eps2f = (float)A.eps2;
+    const double r64 = r64_of(c);
+    A.thr2f = (float)(r64 * r64 + A.eps2);
+    A.SWQ = c->prm.SWQ; A.FLQ = c->prm.FLQ;
+    return A;
+}
+
+int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel)
+{
+    if (c->n_dust == 0) return DDYN_OK;
+    TimedLaunch tl(c, DDYN_KCLASS_DUST);
+    const unsigned blocks = (unsigned)((c->n_dust + kWG - 1) / kWG);
+    const bool f32 = c->prm.precision == DDYN_FP32_FORCE;
+    if (accel) {
+        if (f32) k_dust<true, true><<<blocks, kWG, 0, c->stream>>>(A);
+        else k_dust<false, true><<<blocks, kWG, 0, c->stream>>>(A);
+    } else {
+        if (f32) k_dust<true, false><<<blocks, kWG, 0, c->stream>>>(A);
+        else k_dust<false, false><<<blocks, kWG, 0, c->stream>>>(A);
+    }
+    DDYN_HIP(c, hipGetLastError());
+    return DDYN_OK;
+}
+
+int one_step(ddyn_ctx* c, double dt)
+{
+    int rc;
+    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
+    for (int s = 0; s < 3; ++s) {
+        if ((rc = launch_pairs(c, s))) return rc;
+        if ((rc = launch_stage(c, s, stage_args(c, hs[s])))) return rc;
+    }
+    if ((rc = launch_pairs(c, 3))) return rc;
+    if ((rc = launch_dust(c, dust_args(c, dt), false))) return rc;   // before the final update
+    return launch_stage(c, 3, stage_args(c, dt / 6.0));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ddyn_create(ddyn_ctx** out, int device)
+{
+    if (!out) return DDYN_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DDYN_ERR_HIP;
+    if (device < 0 || device >= ndev) return DDYN_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return DDYN_ERR_HIP;
+    ddyn_ctx* c = new ddyn_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return DDYN_ERR_HIP;
+    }
+    if (dalloc(c, c->beta_t, kMaxBins) || dalloc(c, c->betasw_t, kMaxBins) || dalloc(c, c->q_t, kMaxBins)) {
+        ddyn_destroy(c);
+        return DDYN_ERR_NOMEM;
+    }
+    *out = c;
+    return DDYN_OK;
+}
+
+void ddyn_destroy(ddyn_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drain_events(c);
+    free_state(c);
+    dfree(c->beta_t); dfree(c->betasw_t); dfree(c->q_t);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* ddyn_last_error(const ddyn_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ddyn_set_params(ddyn_ctx* c, const ddyn_params* p)
+{
+    if (!c || !p) return DDYN_ERR_ARG;
+    if (p->n_planets < 1) return fail(c, DDYN_ERR_ARG, "n_planets must count the star (>= 1)");
+    if (p->precision != DDYN_FP64 && p->precision != DDYN_FP32_FORCE)
+        return fail(c, DDYN_ERR_ARG, "unknown precision form");
+    if (!(p->eps >= 0.0) || !(p->Rmin >= 0.0)) return fail(c, DDYN_ERR_ARG, "eps and Rmin must be >= 0");
+    if (p->SWQ == 1 && !(p->vsw > 0.0)) return fail(c, DDYN_ERR_ARG, "SWQ needs vsw > 0");
+    if (c->x && p->n_planets > c->n_grav) return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
+    c->prm = *p;
+    c->have_params = true;
+    if (c->x) {
+        (void)hipSetDevice(c->device);
+        const int old = c->nslice;
+        plan_pairs(c);
+        if (c->nslice > old || (c->nslice && !c->part)) {
+            dfree(c->part);
+            if (dalloc(c, c->part, (size_t)c->nslice * 3 * (size_t)c->n_grav)) return DDYN_ERR_NOMEM;
+        }
+    }
+    return DDYN_OK;
+}
+
+int ddyn_upload_state(ddyn_ctx* c, int64_t n_grav, int64_t n_dust, const double* x, const double* y,
+                      const double* z, const double* vx, const double* vy, const double* vz,
+                      const double* mu, const double* N, const int32_t* id)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (n_grav < 1 || n_dust < 0 || n_grav > (int64_t)1 << 30 || n_dust > (int64_t)1 << 30)
+        return fail(c, DDYN_ERR_ARG, "need 1 <= n_grav and 0 <= n_dust, each below 2^30");
+    if (!x || !y || !z || !vx || !vy || !vz || !mu || !N)
+        return fail(c, DDYN_ERR_ARG, "state arrays must not be NULL");
+    if (c->have_params && c->prm.n_planets > n_grav)
+        return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    free_state(c);
+    const size_t na = (size_t)(n_grav + n_dust), ng = (size_t)n_grav;
+    double** arr[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->mu, &c->N};
+    const double* src[] = {x, y, z, vx, vy, vz, mu, N};
+    for (int a = 0; a < 8; ++a)
+        if (dalloc(c, *arr[a], na)) { free_state(c); return DDYN_ERR_NOMEM; }
+    if (dalloc(c, c->id, na)) { free_state(c); return DDYN_ERR_NOMEM; }
+    for (int s = 0; s < 4; ++s)
+        if (dalloc(c, c->p64[s], ng) || dalloc(c, c->p32[s], ng)) { free_state(c); return DDYN_ERR_NOMEM; }
+    if (dalloc(c, c->stage, 9 * ng) || dalloc(c, c->acc, 3 * na) ||
+        dalloc(c, c->rem, (size_t)(n_dust > 0 ? n_dust : 1)) || dalloc(c, c->beta_p, (size_t)(n_dust > 0 ? n_dust : 1)) ||
+        dalloc(c, c->betasw_p, (size_t)(n_dust > 0 ? n_dust : 1))) {
+        free_state(c);
+        return DDYN_ERR_NOMEM;
+    }
+    c->n_grav = n_grav; c->n_dust = n_dust;
+    for (int a = 0; a < 8; ++a) DDYN_HIP(c, hipMemcpy(*arr[a], src[a], na * sizeof(double), hipMemcpyHostToDevice));
+    if (id) {
+        DDYN_HIP(c, hipMemcpy(c->id, id, na * sizeof(int32_t), hipMemcpyHostToDevice));
+    } else {
+        std::vector<int32_t> m1(na, -1);
+        DDYN_HIP(c, hipMemcpy(c->id, m1.data(), na * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    k_init_pos<<<(unsigned)((ng + kWG - 1) / kWG), kWG, 0, c->stream>>>((int)ng, c->x, c->y, c->z, c->mu,
+                                                                       c->p64[0], c->p32[0]);
+    DDYN_HIP(c, hipGetLastError());
+    if (c->have_params) {
+        plan_pairs(c);
+        if (c->nslice && dalloc(c, c->part, (size_t)c->nslice * 3 * ng)) { free_state(c); return DDYN_ERR_NOMEM; }
+    }
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DDYN_OK;
+}
+
+int ddyn_upload_bin_tables(ddyn_ctx* c, int32_t n_bins, const float* beta, const float* betasw, const float* q)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (n_bins < 0 || n_bins > kMaxBins)
+        return fail(c, DDYN_ERR_ARG, "n_bins must be in [0, 2700] (nsize)");
+    if (n_bins > 0 && (!beta || !betasw || !q)) return fail(c, DDYN_ERR_ARG, "tables must not be NULL");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    if (n_bins > 0) {
+        DDYN_HIP(c, hipMemcpy(c->beta_t, beta, n_bins * sizeof(float), hipMemcpyHostToDevice));
+        DDYN_HIP(c, hipMemcpy(c->betasw_t, betasw, n_bins * sizeof(float), hipMemcpyHostToDevice));
+        DDYN_HIP(c, hipMemcpy(c->q_t, q, n_bins * sizeof(float), hipMemcpyHostToDevice));
+    }
+    c->n_bins = n_bins;
+    return DDYN_OK;
+}
+
+int ddyn_upload_grain_beta(ddyn_ctx* c, const double* beta, const double* betasw)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (!c->x) return fail(c, DDYN_ERR_STATE, "ddyn_upload_state has not been called");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    if (!beta) { c->use_beta_p = false; c->have_betasw_p = false; return DDYN_OK; }
+    const size_t nd = (size_t)c->n_dust;
+    if (nd) {
+        DDYN_HIP(c, hipMemcpy(c->beta_p, beta, nd * sizeof(double), hipMemcpyHostToDevice));
+        if (betasw) DDYN_HIP(c, hipMemcpy(c->betasw_p, betasw, nd * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->use_beta_p = true;
+    c->have_betasw_p = betasw != nullptr;
+    return DDYN_OK;
+}
+
+int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!(dt > 0.0) || nsteps < 0) return fail(c, DDYN_ERR_ARG, "need dt > 0 and nsteps >= 0");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    for (int64_t k = 0; k < nsteps; ++k)
+        if ((rc = one_step(c, dt))) return rc;
+    return DDYN_OK;
+}
+
+int ddyn_accel(ddyn_ctx* c, double* ax, double* ay, double* az, int32_t* removed)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!ax || !ay || !az) return fail(c, DDYN_ERR_ARG, "output arrays must not be NULL");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav;
+    if ((rc = launch_pairs(c, 0))) return rc;
+    StageArgs S = stage_args(c, 0.0);
+    S.ox = c->acc; S.oy = c->acc + na; S.oz = c->acc + 2 * na;
+    if ((rc = launch_stage(c, 4, S))) return rc;
+    DustArgs D = dust_args(c, 0.0);
+    D.ox = c->acc + ng; D.oy = c->acc + na + ng; D.oz = c->acc + 2 * na + ng;
+    D.removed = c->rem;
+    if ((rc = launch_dust(c, D, true))) return rc;
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    DDYN_HIP(c, hipMemcpy(ax, c->acc, na * sizeof(double), hipMemcpyDeviceToHost));
+    DDYN_HIP(c, hipMemcpy(ay, c->acc + na, na * sizeof(double), hipMemcpyDeviceToHost));
+    DDYN_HIP(c, hipMemcpy(az, c->acc + 2 * na, na * sizeof(double), hipMemcpyDeviceToHost));
+    if (removed && c->n_dust)
+        DDYN_HIP(c, hipMemcpy(removed, c->rem, (size_t)c->n_dust * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return DDYN_OK;
+}
+
+int ddyn_download_state(ddyn_ctx* c, double* x, double* y, double* z, double* vx, double* vy,
+                        double* vz, double* N)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (!c->x) return fail(c, DDYN_ERR_STATE, "ddyn_upload_state has not been called");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t bytes = (size_t)(c->n_grav + c->n_dust) * sizeof(double);
+    double* dst[] = {x, y, z, vx, vy, vz, N};
+    const double* src[] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N};
+    for (int a = 0; a < 7; ++a)
+        if (dst[a]) DDYN_HIP(c, hipMemcpy(dst[a], src[a], bytes, hipMemcpyDeviceToHost));
+    return DDYN_OK;
+}
+
+int ddyn_energy(ddyn_ctx* c, double* out7)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out7) return fail(c, DDYN_ERR_ARG, "out7 must not be NULL");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    const size_t ng = (size_t)c->n_grav;
+    k_potential<<<(unsigned)((ng + kWG - 1) / kWG), kWG, 0, c->stream>>>(c->p64[0], (int)ng,
+                                                                        c->prm.n_planets, eps2_of(c), c->acc);
+    DDYN_HIP(c, hipGetLastError());
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double> h(7 * ng);
+    DDYN_HIP(c, hipMemcpy(h.data(), c->acc, ng * sizeof(double), hipMemcpyDeviceToHost));
+    const double* src[] = {c->x, c->y, c->z, c->vx, c->vy, c->vz};
+    for (int a = 0; a < 6; ++a)
+        DDYN_HIP(c, hipMemcpy(h.data() + (a + 1) * ng, src[a], ng * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> m(ng);
+    DDYN_HIP(c, hipMemcpy(m.data(), c->mu, ng * sizeof(double), hipMemcpyDeviceToHost));
+    long double K = 0, U = 0, L[3] = {0, 0, 0}, P[3] = {0, 0, 0};
+    const double *phi = h.data(), *X = phi + ng, *Y = X + ng, *Z = Y + ng, *VX = Z + ng, *VY = VX + ng,
+                 *VZ = VY + ng;
+    for (size_t i = 0; i < ng; ++i) {
+        K += 0.5L * m[i] * ((long double)VX[i] * VX[i] + (long double)VY[i] * VY[i] + (long double)VZ[i] * VZ[i]);
+        U += 0.5L * m[i] * phi[i];
+        L[0] += (long double)m[i] * (Y[i] * VZ[i] - Z[i] * VY[i]);
+        L[1] += (long double)m[i] * (Z[i] * VX[i] - X[i] * VZ[i]);
+        L[2] += (long double)m[i] * (X[i] * VY[i] - Y[i] * VX[i]);
+        P[0] += (long double)m[i] * VX[i]; P[1] += (long double)m[i] * VY[i]; P[2] += (long double)m[i] * VZ[i];
+    }
+    out7[0] = (double)(K + U);
+    for (int a = 0; a < 3; ++a) { out7[1 + a] = (double)L[a]; out7[4 + a] = (double)P[a]; }
+    return DDYN_OK;
+}
+
+int ddyn_synchronize(ddyn_ctx* c)
+{
+    if (!c) return DDYN_ERR_ARG;
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    return DDYN_OK;
+}
+
+int ddyn_set_timing(ddyn_ctx* c, int enable)
+{
+    if (!c) return DDYN_ERR_ARG;
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    drain_events(c);
+    for (int k = 0; k < DDYN_KCLASS_COUNT; ++k) { c->t_launch[k] = 0; c->t_ms[k] = 0.0; }
+    c->timing = enable != 0;
+    return DDYN_OK;
+}
+
+int ddyn_read_timing(ddyn_ctx* c, int32_t kclass, int64_t* launches, double* total_ms)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (kclass < 0 || kclass >= DDYN_KCLASS_COUNT) return fail(c, DDYN_ERR_ARG, "unknown kernel class");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    drain_events(c);
+    if (launches) *launches = c->t_launch[kclass];
+    if (total_ms) *total_ms = c->t_ms[kclass];
+    return DDYN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+"""ctypes binding of the C ABI in include/ddyn.h (libddyn_hip.so).
+
+The library is built in-tree by diskdyn_amd.build (hipcc --offload-arch=gfx950).
+There is no CPU fallback: if the library or a GPU is missing, constructing an
+Engine raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .system import DiskState, Params
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libddyn_hip.so")
+
+DDYN_KCLASS_PAIRS, DDYN_KCLASS_STAGE, DDYN_KCLASS_DUST = 0, 1, 2
+KCLASS_NAMES = {0: "pairs", 1: "stage", 2: "dust"}
+
+# Every entry point declared in include/ddyn.h, with its ctypes signature.
+_D = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_F = ctypes.POINTER(ctypes.c_float)
+_VP = ctypes.c_void_p
+
+
+class DDynParams(ctypes.Structure):
+    _fields_ = [("n_planets", ctypes.c_int32), ("SWQ", ctypes.c_int32), ("FLQ", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("eps", ctypes.c_double), ("Rmin", ctypes.c_double),
+                ("vsw", ctypes.c_double), ("Wz", ctypes.c_double), ("Bstar", ctypes.c_double),
+                ("Rstar", ctypes.c_double), ("light", ctypes.c_double),
+                ("fp64_radius", ctypes.c_double)]
+
+
+SIGNATURES = {
+    "ddyn_create": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int]),
+    "ddyn_destroy": (None, [_VP]),
+    "ddyn_last_error": (ctypes.c_char_p, [_VP]),
+    "ddyn_set_params": (ctypes.c_int, [_VP, ctypes.POINTER(DDynParams)]),
+    "ddyn_upload_state": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_int64] + [_D] * 8 + [_I32]),
+    "ddyn_upload_bin_tables": (ctypes.c_int, [_VP, ctypes.c_int32, _F, _F, _F]),
+    "ddyn_upload_grain_beta": (ctypes.c_int, [_VP, _D, _D]),
+    "ddyn_step": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int64]),
+    "ddyn_accel": (ctypes.c_int, [_VP, _D, _D, _D, _I32]),
+    "ddyn_download_state": (ctypes.c_int, [_VP] + [_D] * 7),
+    "ddyn_energy": (ctypes.c_int, [_VP, _D]),
+    "ddyn_synchronize": (ctypes.c_int, [_VP]),
+    "ddyn_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "ddyn_read_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _I64, _D]),
+}
+
+_lib = None
+
+
+class DDynError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load libddyn_hip.so and declare every C ABI function (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise DDynError(f"{path} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I32)
+
+
+def _fp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
+
+
+def to_c_params(p: Params) -> DDynParams:
+    return DDynParams(int(p.n_planets), int(p.SWQ), int(p.FLQ), int(p.precision), float(p.eps),
+                      float(p.Rmin), float(p.vsw), float(p.Wz), float(p.Bstar), float(p.Rstar),
+                      float(p.light), float(p.fp64_radius))
+
+
+class Engine:
+    """One device context: owns the device state of one DiskState."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.ctx = _VP()
+        rc = self.lib.ddyn_create(ctypes.byref(self.ctx), int(device))
+        if rc != 0:
+            raise DDynError(f"ddyn_create(device={device}) failed with code {rc}")
+        self.n_grav = self.n_dust = 0
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.ddyn_last_error(self.ctx)
+            raise DDynError(f"{what}: code {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.ddyn_destroy(self.ctx)
+            self.ctx = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, p: Params):
+        cp = to_c_params(p)
+        self._check(self.lib.ddyn_set_params(self.ctx, ctypes.byref(cp)), "ddyn_set_params")
+
+    def upload(self, st: DiskState, params: Params = None):
+        st.validate()
+        if params is not None:
+            self.set_params(params)
+        self._check(self.lib.ddyn_upload_state(self.ctx, st.n_grav, st.n_dust, _dp(st.x), _dp(st.y),
+                                               _dp(st.z), _dp(st.vx), _dp(st.vy), _dp(st.vz),
+                                               _dp(st.mu), _dp(st.N), _ip(st.id)),
+                    "ddyn_upload_state")
+        self.n_grav, self.n_dust = st.n_grav, st.n_dust
+        if st.beta_t is not None:
+            nb = len(st.beta_t)
+            self._check(self.lib.ddyn_upload_bin_tables(
+                self.ctx, nb, _fp(np.ascontiguousarray(st.beta_t, np.float32)),
+                _fp(np.ascontiguousarray(st.betasw_t, np.float32)),
+                _fp(np.ascontiguousarray(st.q_t, np.float32))), "ddyn_upload_bin_tables")
+        self._check(self.lib.ddyn_upload_grain_beta(self.ctx, _dp(st.beta_p), _dp(st.betasw_p)),
+                    "ddyn_upload_grain_beta")
+        if params is not None:   # plan the pair split for the new sizes
+            self.set_params(params)
+
+    def step(self, dt, nsteps=1):
+        self._check(self.lib.ddyn_step(self.ctx, float(dt), int(nsteps)), "ddyn_step")
+
+    def synchronize(self):
+        self._check(self.lib.ddyn_synchronize(self.ctx), "ddyn_synchronize")
+
+    def accel(self):
+        n = self.n_grav + self.n_dust
+        ax, ay, az = np.empty(n), np.empty(n), np.empty(n)
+        rem = np.zeros(max(self.n_dust, 1), dtype=np.int32)
+        self._check(self.lib.ddyn_accel(self.ctx, _dp(ax), _dp(ay), _dp(az), _ip(rem)), "ddyn_accel")
+        return np.stack([ax, ay, az], axis=1), rem[: self.n_dust]
+
+    def download(self, st: DiskState = None):
+        n = self.n_grav + self.n_dust
+        out = [np.empty(n) for _ in range(7)]
+        self._check(self.lib.ddyn_download_state(self.ctx, *[_dp(a) for a in out]),
+                    "ddyn_download_state")
+        if st is None:
+            return out
+        st.x, st.y, st.z, st.vx, st.vy, st.vz, st.N = out
+        return st
+
+    def energy(self):
+        out = np.zeros(7)
+        self._check(self.lib.ddyn_energy(self.ctx, _dp(out)), "ddyn_energy")
+        return out
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.ddyn_set_timing(self.ctx, 1 if enable else 0), "ddyn_set_timing")
+
+    def read_timing(self, kclass):
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0.0)
+        self._check(self.lib.ddyn_read_timing(self.ctx, int(kclass), ctypes.byref(n), ctypes.byref(ms)),
+                    "ddyn_read_timing")
+        return n.value, ms.value
+
+
+def run(workload, device=0, nsteps=None, precision=None):
+    """Upload a workload, step it and return the final DiskState (convenience)."""
+    st = workload.state.copy()
+    prm = workload.params if precision is None else workload.params.with_precision(precision)
+    with Engine(device) as eng:
+        eng.upload(st, prm)
+        eng.step(workload.dt, workload.nsteps if nsteps is None else nsteps)
+        eng.download(st)
+    return st

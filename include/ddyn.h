@@ -1,0 +1,133 @@
+/*
+ * ddyn.h -- C ABI of the MI355X-native DiskDyn dynamical hot path.
+ *
+ * The reference drives this path through three Grid member calls made once
+ * per time step by Grid::evolve (src/grid.cu:387-389):
+ *     massive_RK4(dt)   src/grid.cu:693   four massive-body RK4 stages
+ *     small_RK4(dt)     src/grid.cu:729   fused four-stage grain RK4
+ *     massive_final(dt) src/grid.cu:742   massive-body RK4 combination
+ * on the device SoA `coord gpu` (src/ddyn.cuh:44-58) that Grid allocates and
+ * fills in allocgpu/sync_to_gpu/sync_dust_to_gpu (src/init.cu:543-667) and
+ * reads back in sync_from_gpu (src/init.cu:670-682).  The reference has no
+ * plugin or FFI layer; the functions below replace that member-call boundary
+ * one for one, with plain pointers and sizes, so a C, C++, Python (ctypes) or
+ * any other FFI caller can drive the path.
+ *
+ * Units follow the reference: AU, yr, Msun, and every mass is stored as
+ * mu = G*M with G = 4*pi^2 (src/ddyn.cuh:30, src/read_in.cu:76).  Particle
+ * order in every array is [star (0) | planets | swarm | dust], exactly as in
+ * the reference (n_grav = star + planets + swarm, src/grid.cu:696).
+ *
+ * Conventions: functions return DDYN_OK (0) or a negative DDYN_ERR_* code and
+ * never exit the process; ddyn_last_error() gives the message.  The caller owns
+ * every host array (copied in upload/download); the context owns all device
+ * memory and one HIP stream.  A context is not thread safe.  ddyn_step() is
+ * asynchronous on the context stream; download, accel and energy synchronise.
+ */
+#ifndef DDYN_H
+#define DDYN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDYN_OK 0
+#define DDYN_ERR_ARG (-1)     /* bad argument or size */
+#define DDYN_ERR_HIP (-2)     /* HIP runtime failure */
+#define DDYN_ERR_STATE (-3)   /* call out of order (e.g. step before upload) */
+#define DDYN_ERR_NOMEM (-4)   /* device allocation failed */
+
+/* Force precision forms. */
+#define DDYN_FP64 0           /* every term in FP64 (the plain form) */
+#define DDYN_FP32_FORCE 1     /* swarm-sourced pairs in FP32 with FP64 per-tile flushes;
+                                 star/planet terms, close pairs and RK4 state in FP64 */
+
+typedef struct ddyn_ctx ddyn_ctx;
+
+/* Physics switches and constants.  Replaces the cconfig globals and the
+ * scalars that small_RK4 passes to RK4calc_dust (src/grid.cu:737). */
+typedef struct {
+    int32_t n_planets;   /* star + planets (cfg->n_planets counts the star, src/read_in.cu:63-79) */
+    int32_t SWQ;         /* 1: stellar-wind drag on (.param record 10) */
+    int32_t FLQ;         /* 1: Lorentz force on (.param record 10) */
+    int32_t precision;   /* DDYN_FP64 or DDYN_FP32_FORCE */
+    double  eps;         /* Plummer softening [AU] for swarm-sourced pairs; 0 = reference law (none) */
+    double  Rmin;        /* grain sublimation radius [AU] (Grid::Rmin, src/grid.cu:737) */
+    double  vsw;         /* stellar-wind speed [AU/yr] (cfg->vsw*year/AU, src/grid.cu:737) */
+    double  Wz;          /* stellar rotation rate [1/yr] (cfg->Wz) */
+    double  Bstar;       /* stellar surface field in the reference's units (cfg->Bstar) */
+    double  Rstar;       /* stellar radius [AU] (cfg->rstar) */
+    double  light;       /* speed of light [AU/yr]; 0 selects lightAU = 63239.6717367 */
+    double  fp64_radius; /* DDYN_FP32_FORCE: swarm pairs closer than this [AU] are evaluated in
+                            FP64; <= 0 selects 10*eps */
+} ddyn_params;
+
+/* Create a context on HIP device `device` (replaces checkcuda, src/cuda.cu:92-98). */
+int  ddyn_create(ddyn_ctx **out, int device);
+/* Free all device memory (replaces the Grid destructor frees, src/init.cu:111-215). */
+void ddyn_destroy(ddyn_ctx *ctx);
+/* Message of the last failure on this context ("" if none). */
+const char *ddyn_last_error(const ddyn_ctx *ctx);
+
+/* Set physics and precision (replaces the cconfig fields read by the kernels). */
+int ddyn_set_params(ddyn_ctx *ctx, const ddyn_params *p);
+
+/* Upload the particle state (replaces allocgpu + null_gpu + sync_to_gpu,
+ * src/init.cu:543-637).  Arrays have n_grav + n_dust entries in reference
+ * order; mu = G*M; N is the tracer count (0 = removed, skipped forever,
+ * src/gpu_functions.cu:643); id is the grain size-bin index (-1 = none;
+ * may be NULL for all -1). */
+int ddyn_upload_state(ddyn_ctx *ctx, int64_t n_grav, int64_t n_dust,
+                      const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz,
+                      const double *mu, const double *N, const int32_t *id);
+
+/* Per-size-bin beta, beta_sw and charge tables, float as in the reference's
+ * __constant__ arrays (src/ddyn.cu:30-32, filled by sync_dust_to_gpu,
+ * src/init.cu:639-667).  n_bins <= 2700 (nsize, src/ddyn.cuh:35). */
+int ddyn_upload_bin_tables(ddyn_ctx *ctx, int32_t n_bins,
+                           const float *beta, const float *betasw, const float *q);
+
+/* Per-grain FP64 beta and beta_sw (n_dust each), overriding the bin tables.
+ * No reference counterpart: the reference only has per-bin float tables.
+ * betasw may be NULL (zeros).  beta == NULL removes the override. */
+int ddyn_upload_grain_beta(ddyn_ctx *ctx, const double *beta, const double *betasw);
+
+/* Advance nsteps RK4 steps of size dt [yr]; one step is
+ * massive_RK4(dt) + small_RK4(dt) + massive_final(dt) (src/grid.cu:387-389). */
+int ddyn_step(ddyn_ctx *ctx, double dt, int64_t nsteps);
+
+/* One force evaluation at the current state (stage-1 forces of the next step):
+ * ax/ay/az get n_grav + n_dust accelerations [AU/yr^2]; removed (may be NULL)
+ * gets n_dust flags (1 = the grain would be removed: sublimation or exact hit).
+ * No reference counterpart (the reference never exposes accelerations). */
+int ddyn_accel(ddyn_ctx *ctx, double *ax, double *ay, double *az, int32_t *removed);
+
+/* Download the state (replaces sync_from_gpu, src/init.cu:670-682).  Unlike
+ * the reference, N is downloaded too so removals are visible.  Any pointer may
+ * be NULL to skip that array. */
+int ddyn_download_state(ddyn_ctx *ctx, double *x, double *y, double *z,
+                        double *vx, double *vy, double *vz, double *N);
+
+/* Invariants of the massive set in mu units: out7 = {E, Lx, Ly, Lz, Px, Py, Pz}
+ * with E = sum 0.5 mu v^2 - sum_{i<j} mu_i mu_j / sqrt(r^2 + eps_ij^2).
+ * No reference counterpart. */
+int ddyn_energy(ddyn_ctx *ctx, double *out7);
+
+/* Wait for all work on the context stream. */
+int ddyn_synchronize(ddyn_ctx *ctx);
+
+/* Kernel timing with HIP events on the context stream.  Classes: */
+#define DDYN_KCLASS_PAIRS 0   /* massive swarm-pair kernel */
+#define DDYN_KCLASS_STAGE 1   /* massive stage epilogue (direct terms + RK4 update) */
+#define DDYN_KCLASS_DUST 2    /* fused four-stage grain kernel */
+#define DDYN_KCLASS_COUNT 3
+int ddyn_set_timing(ddyn_ctx *ctx, int enable);
+/* Totals since timing was enabled (synchronises). */
+int ddyn_read_timing(ddyn_ctx *ctx, int32_t kclass, int64_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,58 @@
+/*
+ * ddyn_oracle.h -- FP64 CPU oracle for DiskDyn's dynamical hot path.
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ */
+#ifndef DDYN_ORACLE_H
+#define DDYN_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physics switches and constants of one run (units AU, yr, Msun; mu = G*M). */
+typedef struct {
+    int32_t n_planets;   /* star + planets, as cfg->n_planets (src/read_in.cu:63-79) */
+    int32_t SWQ;         /* stellar-wind drag on (record 10) */
+    int32_t FLQ;         /* Lorentz force on (record 10) */
+    int32_t pad_;
+    double  eps;         /* Plummer softening [AU] on swarm-sourced pairs; 0 = reference law */
+    double  Rmin;        /* sublimation radius [AU] */
+    double  vsw;         /* stellar-wind speed [AU/yr] (grid.cu:737 passes vsw*year/AU) */
+    double  Wz;          /* stellar rotation [1/yr] */
+    double  Bstar;       /* stellar field, reference units (read_in.cu:597) */
+    double  Rstar;       /* stellar radius [AU] */
+    double  light;       /* speed of light [AU/yr] (lightAU, ddyn.cuh:22) */
+} orc_params;
+
+/* Particle state in the reference order [star | planets | swarm | dust]. */
+typedef struct {
+    int64_t n_grav, n_dust;
+    double *x, *y, *z, *vx, *vy, *vz, *mu, *N;   /* n_grav + n_dust each */
+    const int32_t *id;                            /* size-bin id (may be NULL: all -1) */
+    int32_t n_bins;
+    int32_t pad_;
+    const float *beta_tab, *betasw_tab, *q_tab;   /* per-bin tables (may be NULL) */
+    const double *beta_p, *betasw_p;              /* per-grain overrides (may be NULL) */
+} orc_state;
+
+void orc_massive_accel(const orc_params *p, int64_t n_grav,
+                       const double *x, const double *y, const double *z, const double *mu,
+                       double *ax, double *ay, double *az);
+int  orc_dust_accel(const orc_params *p, const orc_state *s, double *ax, double *ay, double *az,
+                    int32_t *removed);
+int  orc_step(const orc_params *p, orc_state *s, double dt, int64_t nsteps);
+void orc_energy(const orc_params *p, int64_t n, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *mu,
+                double *out7);
+void orc_coe2rv(double M, double m, double a, double e, double inc, double w, double o,
+                double nu, double beta, double *out6);
+void orc_recalc_com(int64_t n_planets, double *x, double *y, double *z,
+                    double *vx, double *vy, double *vz, const double *mu);
+int  orc_max_threads(void);
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
