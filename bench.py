@@ -1,0 +1,199 @@
+"""Throughput of the dynamical hot path on one MI355X.
+
+Default workload: the self-gravitating swarm (1 Msun star + 65,536 bodies,
+masses log-uniform in [1e-10, 1e-7] Msun, a~U(20,40) AU, Plummer eps = 1e-3 AU,
+dt = 0.02 yr) integrated with RK4 in the FP32-force / FP64-state form; the
+plain FP64 form runs the same steps for the speed ratio.  A "step" is one RK4
+step (four force evaluations) of the whole system with the state resident in
+HBM.  Metric: directed pairwise interactions per second,
+n_grav (n_grav - 1) + n_dust n_grav per evaluation, four evaluations per step.
+
+The roofline entry is the massive swarm-pair kernel: 20 FLOP per directed
+interaction (FMA = 2, rsqrt = 1) over its mean launch time measured with HIP
+events on the library's stream, against the FP32 vector peak.  The CPU
+baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from diskdyn_amd import synth  # noqa: E402
+from diskdyn_amd.engine import DDYN_KCLASS_DUST, DDYN_KCLASS_PAIRS, Engine  # noqa: E402
+from diskdyn_amd.system import FP32_FORCE, FP64  # noqa: E402
+
+FLOP_PER_INTERACTION = 20
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def workload(name, precision):
+    if name == "cfg2":
+        w = synth.config2(precision=precision)
+        desc = "self-gravitating swarm: 1 Msun star + 65,536 bodies, eps=1e-3 AU, dt=0.02 yr"
+    elif name == "cfg3":
+        w = synth.config3(precision=precision)
+        desc = ("dusty debris ring: 0.5 Msun star, 3e-5 Msun planet, 1,024 planetesimals, "
+                "262,144 grains with radiation/PR/wind drag, dt=0.05 yr")
+    elif name == "cfg5":
+        w = synth.config5(precision=precision)
+        desc = "large self-gravitating disk: 1 Msun star + 1,048,576 bodies, eps=5e-4 AU, dt=0.02 yr"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return w, desc
+
+
+def timed_run(w, precision, steps, warmup):
+    """Warm up, then time `steps` RK4 steps; returns seconds and kernel timings."""
+    with Engine(0) as eng:
+        eng.upload(w.state.copy(), w.params.with_precision(precision))
+        if warmup:
+            eng.step(w.dt, warmup)
+        eng.synchronize()
+        eng.set_timing(True)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.step(w.dt, steps)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        kt = {k: eng.read_timing(k) for k in (DDYN_KCLASS_PAIRS, DDYN_KCLASS_DUST)}
+        a, _ = eng.accel()
+    return t1 - t0, kt, a
+
+
+def force_error(w, a_gpu, nsample=2048, seed=7):
+    """Median/max relative acceleration error of the GPU against the FP64
+    oracle on sampled massive targets (grains: all of a sampled subset)."""
+    from oracle import oracle as O
+    st = w.state
+    rng = np.random.default_rng(seed)
+    ng = st.n_grav
+    idx = np.sort(rng.choice(ng, size=min(nsample, ng), replace=False))
+    ref = O.massive_accel_targets(w.params, st.x[:ng], st.y[:ng], st.z[:ng], st.mu[:ng], idx)
+    err = np.linalg.norm(a_gpu[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    if st.n_dust:
+        a_ref, _ = O.accel(w.params, st)
+        e2 = np.linalg.norm(a_gpu[ng:] - a_ref[ng:], axis=1) / np.linalg.norm(a_ref[ng:], axis=1)
+        err = np.concatenate([err, e2])
+    return float(np.median(err)), float(err.max())
+
+
+def cpu_baseline(w, target_s=12.0):
+    """FP64 oracle on this host's cores: a bounded sample of force evaluations.
+
+    Massive targets are evaluated in chunks (each over all n_grav sources) until
+    about target_s seconds have passed; the rate is interactions/s."""
+    from oracle import oracle as O
+    st = w.state
+    ng, nd = st.n_grav, st.n_dust
+    threads = O.max_threads()
+    done = 0
+    t0 = time.perf_counter()
+    if nd == 0:
+        chunk = max(256, min(ng, 4096))
+        i0 = 0
+        while time.perf_counter() - t0 < target_s:
+            idx = np.arange(i0, min(i0 + chunk, ng))
+            O.massive_accel_targets(w.params, st.x[:ng], st.y[:ng], st.z[:ng], st.mu[:ng], idx)
+            done += len(idx) * (ng - 1)
+            i0 = (i0 + chunk) % ng
+        sample = f"massive accelerations of {done // (ng - 1)} targets x {ng} sources (FP64)"
+    else:
+        sub = w.state.copy()
+        steps = 0
+        while time.perf_counter() - t0 < target_s:
+            O.step(w.params, sub, w.dt, 1)
+            steps += 1
+        done = steps * 4 * w.interactions_per_eval()
+        sample = f"{steps} full FP64 RK4 steps"
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
+            "sample": sample + f" in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="cfg2", choices=["cfg2", "cfg3", "cfg5"])
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="steps for the plain FP64 comparison run (default: --steps)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    w32, desc = workload(args.workload, FP32_FORCE)
+    per_step = 4 * w32.interactions_per_eval()
+
+    t32, kt32, a32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup)
+    k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
+    t64, kt64, a64 = timed_run(w32, FP64, k64, min(args.warmup, 2))
+
+    med32, max32 = force_error(w32, a32)
+    med64, max64 = force_error(w32, a64)
+
+    value = per_step * args.steps / t32
+    v64 = per_step * k64 / t64
+    # dominant kernel: massive pairs (swarm workloads) or the grain kernel (debris ring)
+    dom = DDYN_KCLASS_PAIRS if w32.state.n_dust == 0 else DDYN_KCLASS_DUST
+    nl, ms = kt32[dom]
+    if dom == DDYN_KCLASS_PAIRS:
+        ns = w32.state.n_grav - w32.params.n_planets
+        inter_launch = w32.state.n_grav * ns
+        kname = "k_pairs32"
+    else:
+        inter_launch = 4 * w32.state.n_dust * w32.state.n_grav
+        kname = "k_dust<fp32>"
+    avg_ms = ms / max(nl, 1)
+    achieved = FLOP_PER_INTERACTION * inter_launch / (avg_ms * 1e-3) / 1e12
+    nl64, ms64 = kt64[dom]
+    ach64 = FLOP_PER_INTERACTION * inter_launch / (ms64 / max(nl64, 1) * 1e-3) / 1e12
+
+    out = {
+        "metric": "pairwise interactions/s (FP32-force/FP64-state RK4)",
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t32 / args.steps,
+        "steps_per_s": args.steps / t32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair forces, fp64 state",
+        "data": "synthetic (seeded Philox)",
+        "config": {"workload": f"{w32.name}: {desc}", "n_grav": w32.state.n_grav,
+                   "n_dust": w32.state.n_dust, "interactions_per_step": per_step,
+                   "parallelism": "single GPU"},
+        "force_rel_err": {"median": med32, "max": max32, "vs": "FP64 CPU oracle, sampled targets"},
+        "fp64": {"value": v64, "ms_per_step": 1e3 * t64 / k64, "steps": k64,
+                 "force_rel_err_median": med64, "force_rel_err_max": max64,
+                 "kernel_tflops": ach64, "kernel_frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS},
+        "speedup_fp32_vs_fp64": value / v64,
+        "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "launches": nl, "avg_launch_ms": avg_ms,
+                     "flop_per_launch": FLOP_PER_INTERACTION * inter_launch},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(w32)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -91,6 +91,17 @@ void orc_massive_accel(const orc_params *p, int64_t n_grav,
         massive_accel_one(p, n_grav, i, x, y, z, mu, &ax[i], &ay[i], &az[i]);
 }
 
+/* Accelerations of the targets idx[0..nt) only, each summed over all n_grav
+ * sources (sampled parity checks and the bounded CPU baseline). */
+void orc_massive_accel_targets(const orc_params *p, int64_t n_grav,
+                               const double *x, const double *y, const double *z, const double *mu,
+                               int64_t nt, const int64_t *idx, double *ax, double *ay, double *az)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nt; k++)
+        massive_accel_one(p, n_grav, idx[k], x, y, z, mu, &ax[k], &ay[k], &az[k]);
+}
+
 /* Grain acceleration at one RK stage (src/gpu_functions.cu:679-735 for stage
  * 1; stages 2-4 repeat it with the stage position and velocity).
  * (sx,sy,sz) are the massive bodies' positions at this stage, body 0 = star.

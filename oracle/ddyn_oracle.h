@@ -39,6 +39,9 @@ typedef struct {
 void orc_massive_accel(const orc_params *p, int64_t n_grav,
                        const double *x, const double *y, const double *z, const double *mu,
                        double *ax, double *ay, double *az);
+void orc_massive_accel_targets(const orc_params *p, int64_t n_grav,
+                               const double *x, const double *y, const double *z, const double *mu,
+                               int64_t nt, const int64_t *idx, double *ax, double *ay, double *az);
 int  orc_dust_accel(const orc_params *p, const orc_state *s, double *ax, double *ay, double *az,
                     int32_t *removed);
 int  orc_step(const orc_params *p, orc_state *s, double dt, int64_t nsteps);

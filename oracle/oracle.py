@@ -50,6 +50,9 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_massive_accel.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int64] + [_D] * 7
         L.orc_massive_accel.restype = None
+        L.orc_massive_accel_targets.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int64] + [_D] * 4 + \
+            [ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)] + [_D] * 3
+        L.orc_massive_accel_targets.restype = None
         L.orc_dust_accel.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcState), _D, _D, _D, _I32]
         L.orc_dust_accel.restype = ctypes.c_int
         L.orc_step.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcState), ctypes.c_double,
@@ -103,6 +106,18 @@ def massive_accel(p, x, y, z, mu):
     ax, ay, az = np.empty(n), np.empty(n), np.empty(n)
     c = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z, mu)]
     lib().orc_massive_accel(ctypes.byref(params(p)), n, *[_p(v) for v in c], _p(ax), _p(ay), _p(az))
+    return np.stack([ax, ay, az], axis=1)
+
+
+def massive_accel_targets(p, x, y, z, mu, idx):
+    """Accelerations of the massive targets idx (each over all sources)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    nt = len(idx)
+    ax, ay, az = np.empty(nt), np.empty(nt), np.empty(nt)
+    c = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z, mu)]
+    lib().orc_massive_accel_targets(ctypes.byref(params(p)), len(c[0]), *[_p(v) for v in c], nt,
+                                    idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                    _p(ax), _p(ay), _p(az))
     return np.stack([ax, ay, az], axis=1)
 
 
