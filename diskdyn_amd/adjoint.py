@@ -1,0 +1,125 @@
+"""Gradients of trajectory losses through the GPU discrete adjoint.
+
+`Trajectory.apply(r0, v0, mu, beta)` integrates nsteps RK4 steps on the GPU
+(ddyn_step) and returns the final positions and velocities; its backward pass
+calls ddyn_adjoint (checkpoints every `ckpt_every` steps, recomputation in
+between).  Inputs are float64 CPU tensors: r0, v0 (n_all, 3), mu (n_grav,)
+the massive G*M, beta (n_dust,) the per-grain radiation-pressure ratio.
+
+`planet_initial_state` rebuilds the star + planet initial state from planet
+(m, a, e) with the algebra of coe2rv (src/functions.cu:6-46) and recalc_COM
+(src/read_in.cu:344-375) written in torch, so dL/d(m, a, e) follows by
+autograd from the adjoint's dL/d(initial state) and dL/dmu.
+"""
+import math
+
+import numpy as np
+import torch
+
+from .engine import Engine
+from .units import GRAV
+
+
+class Runner:
+    """Fixed parts of a run: the template state (ids, N, beta_sw, grain masses),
+    parameters, step size, step count and checkpoint interval."""
+
+    def __init__(self, state, params, dt, nsteps, ckpt_every=0, device=0):
+        self.state, self.params = state, params
+        self.dt, self.nsteps, self.ckpt_every, self.device = dt, nsteps, ckpt_every, device
+
+    def make_state(self, r0, v0, mu, beta):
+        st = self.state.copy()
+        ng = st.n_grav
+        r = r0.detach().cpu().numpy()
+        v = v0.detach().cpu().numpy()
+        st.x, st.y, st.z = (np.ascontiguousarray(r[:, k]) for k in range(3))
+        st.vx, st.vy, st.vz = (np.ascontiguousarray(v[:, k]) for k in range(3))
+        st.mu[:ng] = mu.detach().cpu().numpy()
+        if st.n_dust:
+            st.beta_p = np.ascontiguousarray(beta.detach().cpu().numpy(), dtype=np.float64)
+            if st.betasw_p is None:
+                st.betasw_p = np.zeros(st.n_dust)
+        return st
+
+    def forward(self, st):
+        with Engine(self.device) as eng:
+            eng.upload(st, self.params)
+            eng.step(self.dt, self.nsteps)
+            out = eng.download(st.copy())
+        return out
+
+    def adjoint(self, st, lam_r, lam_v):
+        with Engine(self.device) as eng:
+            eng.upload(st, self.params)
+            return eng.adjoint(self.dt, self.nsteps, lam_r, lam_v, self.ckpt_every)
+
+
+class Trajectory(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, r0, v0, mu, beta, runner):
+        st = runner.make_state(r0, v0, mu, beta)
+        out = runner.forward(st)
+        ctx.runner = runner
+        ctx.state = st
+        rT = torch.from_numpy(out.pos()).to(r0.dtype)
+        vT = torch.from_numpy(out.vel()).to(v0.dtype)
+        return rT, vT
+
+    @staticmethod
+    def backward(ctx, g_rT, g_vT):
+        lam_r = np.zeros(ctx.state.pos().shape) if g_rT is None else g_rT.detach().cpu().numpy()
+        lam_v = np.zeros(ctx.state.pos().shape) if g_vT is None else g_vT.detach().cpu().numpy()
+        g_r, g_v, g_mu, g_beta = ctx.runner.adjoint(ctx.state, lam_r, lam_v)
+        return (torch.from_numpy(g_r), torch.from_numpy(g_v), torch.from_numpy(g_mu),
+                torch.from_numpy(np.ascontiguousarray(g_beta)), None)
+
+
+def coe2rv_t(M, m, a, e, inc, w, o, nu, beta=0.0):
+    """Torch restatement of coe2rv (src/functions.cu:6-46); returns (r, v) 3-vectors."""
+    p = a * (1.0 - e * e)
+    mu = (M + m) * (1.0 - beta)
+    cnu, snu = torch.cos(nu), torch.sin(nu)
+    xe = p * cnu / (1.0 + e * cnu)
+    ye = p * snu / (1.0 + e * cnu)
+    sq = torch.sqrt(mu / p)
+    vxe = -sq * snu
+    vye = sq * (e + cnu)
+    co, so, cw, sw, ci, si = (torch.cos(o), torch.sin(o), torch.cos(w), torch.sin(w), torch.cos(inc),
+                              torch.sin(inc))
+    r11, r12 = co * cw - so * sw * ci, -co * sw - so * cw * ci
+    r21, r22 = so * cw + co * sw * ci, -so * sw + co * cw * ci
+    r31, r32 = sw * si, cw * si
+    r = torch.stack([xe * r11 + ye * r12, xe * r21 + ye * r22, xe * r31 + ye * r32])
+    v = torch.stack([vxe * r11 + vye * r12, vxe * r21 + vye * r22, vxe * r31 + vye * r32])
+    return r, v
+
+
+def planet_initial_state(mstar, m_p, a_p, e_p, inc=0.0, w=0.0, o=0.0, nu=0.0):
+    """Star + one planet after recalc_COM, as torch tensors (2, 3) and mu (2,).
+
+    mstar, m_p in Msun (m_p, a_p, e_p may require grad)."""
+    dt = torch.float64
+    f = lambda v: v if torch.is_tensor(v) else torch.tensor(float(v), dtype=dt)
+    m_p, a_p, e_p = f(m_p), f(a_p), f(e_p)
+    mu0 = torch.tensor(mstar * GRAV, dtype=dt)
+    mu1 = m_p * GRAV
+    rp, vp = coe2rv_t(mu0, mu1, a_p, e_p, f(inc), f(w), f(o), f(nu))
+    mu = torch.stack([mu0, mu1])
+    r = torch.stack([torch.zeros(3, dtype=dt), rp])
+    v = torch.stack([torch.zeros(3, dtype=dt), vp])
+    # recalc_COM: shift to the barycentre, star velocity zeroes the momentum
+    com = (mu[:, None] * r).sum(0) / mu.sum()
+    r = r - com
+    v0 = -(mu1 * vp) / mu0
+    v = torch.stack([v0, vp])
+    return r, v, mu
+
+
+def ring_loss(rT, target, n_grav):
+    """Mean squared distance of the final grain positions from a target."""
+    d = rT[n_grav:] - target[n_grav:]
+    return (d * d).sum(1).mean()
+
+
+__all__ = ["Runner", "Trajectory", "coe2rv_t", "planet_initial_state", "ring_loss", "math"]

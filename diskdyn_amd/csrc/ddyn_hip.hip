@@ -20,6 +20,7 @@
 // are needed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -523,6 +524,332 @@ __global__ __launch_bounds__(kWG) void k_dust(DustArgs A)
 }
 
 // ---------------------------------------------------------------------------
+// Discrete adjoint of the RK4 step
+// ---------------------------------------------------------------------------
+// Stage map of one step: R1 = r, V1 = v; R2 = r + h/2 V1, V2 = v + h/2 A1;
+// R3 = r + h/2 V2, V3 = v + h/2 A2; R4 = r + h V3, V4 = v + h A3;
+// r' = r + h/6 (V1 + 2V2 + 2V3 + V4), v' = v + h/6 (A1 + 2A2 + 2A3 + A4).
+// Reverse: Vbar_s = w_s rbar', Abar_s = w_s vbar' (w = h/6, h/3, h/3, h/6), then
+// for s = 4..1: Rbar_s = J_R^T Abar_s, Vbar_s += J_V^T Abar_s, rbar += Rbar_s,
+// vbar += Vbar_s, Vbar_{s-1} += h'_s Rbar_s, Abar_{s-1} += h'_s Vbar_s.
+// With K(d) = d/s^3, s^2 = |d|^2 + eps^2, J(d) = dK/dd = (I s^2 - 3 d d^T)/s^5:
+//   massive: Rbar_k = sum_j J(R_j - R_k)(mu_k Abar_j - mu_j Abar_k),
+//            mubar_k = sum_i Abar_i . K(R_k - R_i);
+//   grains:  star term differentiated in closed form (gravity, radiation
+//            pressure, PR and wind drag), pulls mu_j K(R_j - R_t).
+
+// Massive VJP, all pairs: targets k (one per lane), sources j in slice y of
+// [0, n_grav).  Writes part[(y*4 + c) * n + k], c = x, y, z, mu.
+__global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ pos,
+                                                   const double* __restrict__ Ab, int n_grav,
+                                                   int n_planets, int slice_len, double eps2,
+                                                   double* __restrict__ part)
+{
+    __shared__ double4 tp[kTile];
+    __shared__ double4 ta[kTile];
+    const size_t n = (size_t)n_grav;
+    const int k = blockIdx.x * kWG + threadIdx.x;
+    const bool ok = k < n_grav;
+    const double4 pk = ok ? pos[k] : make_double4(kFar, kFar, kFar, 0.0);
+    const double akx = ok ? Ab[k] : 0.0, aky = ok ? Ab[n + k] : 0.0, akz = ok ? Ab[2 * n + k] : 0.0;
+    const double e2k = (k >= n_planets) ? eps2 : 0.0;
+    const int jbeg = blockIdx.y * slice_len;
+    const int jend = min(jbeg + slice_len, n_grav);
+    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
+    for (int j0 = jbeg; j0 < jend; j0 += kTile) {
+        const int tl = min(kTile, jend - j0);
+        __syncthreads();
+        if (threadIdx.x < tl) {
+            const int j = j0 + threadIdx.x;
+            tp[threadIdx.x] = pos[j];
+            ta[threadIdx.x] = make_double4(Ab[j], Ab[n + j], Ab[2 * n + j], 0.0);
+        }
+        __syncthreads();
+        for (int jj = 0; jj < tl; ++jj) {
+            const int j = j0 + jj;
+            const double4 pj = tp[jj];
+            const double4 aj = ta[jj];
+            const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
+            const double s2 = dx * dx + dy * dy + dz * dz + ((j >= n_planets) ? e2k : 0.0);
+            if (s2 > 0.0) {
+                const double inv = 1.0 / __builtin_sqrt(s2);
+                const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+                const double wx = pk.w * aj.x - pj.w * akx, wy = pk.w * aj.y - pj.w * aky,
+                             wz = pk.w * aj.z - pj.w * akz;
+                const double dw = dx * wx + dy * wy + dz * wz;
+                gx += wx * inv3 - 3.0 * dx * dw * inv5;
+                gy += wy * inv3 - 3.0 * dy * dw * inv5;
+                gz += wz * inv3 - 3.0 * dz * dw * inv5;
+                gm -= (aj.x * dx + aj.y * dy + aj.z * dz) * inv3;
+            }
+        }
+    }
+    if (ok) {
+        double* p = part + (size_t)blockIdx.y * 4 * n;
+        p[k] = gx; p[n + k] = gy; p[2 * n + k] = gz; p[3 * n + k] = gm;
+    }
+}
+
+struct AdjArgs {
+    int n_grav, n_planets, n_dust, nslice, nwaves;
+    size_t n_all;
+    const double4* p64[4];
+    const double* x; const double* y; const double* z;
+    const double* vx; const double* vy; const double* vz;
+    const double* mu;
+    const double* N;
+    const int32_t* id;
+    const double *beta_p, *betasw_p;
+    const float *beta_t, *betasw_t;
+    int n_bins, SWQ;
+    double dt, eps2, Rmin2, vsw, light;
+    double* rb;          // 3 * n_all, SoA blocks: dL/dr
+    double* vb;          // 3 * n_all: dL/dv
+    double* Vbm;         // 4 stages * 3 * n_grav
+    double* Abm;         // 4 stages * 3 * n_grav
+    double* Rbg;         // 4 stages * 3 * n_grav: grain pulls on massive stage positions
+    double* mug;         // n_grav: grain pulls on massive masses (this step)
+    double* wpart;       // per-wave partials: nwaves * 4 stages * n_grav * 4
+    const double* part;  // VJP pair partials
+    double* g_mu;        // n_grav accumulated
+    double* g_beta;      // n_dust accumulated
+};
+
+__global__ void k_adj_init(AdjArgs A)
+{
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= A.n_grav) return;
+    const double h = A.dt;
+    const double w[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0};
+    const size_t n = (size_t)A.n_grav, na = A.n_all;
+    for (int st = 0; st < 4; ++st)
+        for (int c = 0; c < 3; ++c) {
+            A.Vbm[(st * 3 + c) * n + i] = w[st] * A.rb[c * na + i];
+            A.Abm[(st * 3 + c) * n + i] = w[st] * A.vb[c * na + i];
+        }
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Grain reverse: one grain per lane; recomputes its four FP64 stages (as k_dust
+// does in the FP64 form), then runs the stage reverse in registers.  The pull
+// of the grain on each massive body is summed per wave and written to wpart.
+__global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
+{
+    const int g = blockIdx.x * kWG + threadIdx.x;
+    const int wave = (blockIdx.x * kWG + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)A.n_grav + g;
+    const size_t na = A.n_all;
+    bool live = g < A.n_dust && A.N[t] != 0.0;
+    double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, bet = 0.0, betsw = 0.0, m = 1.0;
+    if (g < A.n_dust) {
+        x0[0] = A.x[t]; x0[1] = A.y[t]; x0[2] = A.z[t];
+        v0[0] = A.vx[t]; v0[1] = A.vy[t]; v0[2] = A.vz[t];
+        m = A.mu[t];
+        const int32_t id = A.id[t];
+        if (A.beta_p) {
+            bet = A.beta_p[g];
+            betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+        } else if (id >= 0 && id < A.n_bins) {
+            bet = (double)A.beta_t[id];
+            betsw = (double)A.betasw_t[id];
+        }
+    }
+    const double dt = A.dt, dt2 = dt / 2.0;
+    const double hh[3] = {dt2, dt2, dt};
+    double R[4][3], V[4][3];
+    bool gone = false;
+    DustArgs D{};
+    D.Rmin2 = A.Rmin2; D.SWQ = A.SWQ; D.FLQ = 0; D.vsw = A.vsw; D.light = A.light;
+    for (int c = 0; c < 3; ++c) { R[0][c] = x0[c]; V[0][c] = v0[c]; }
+    // forward stages (src/gpu_functions.cu:679-869 in the FP64 form of k_dust)
+    for (int st = 0; st < 4; ++st) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (live && !gone) {
+            const double4* P = A.p64[st];
+            gone = star_term(D, P[0], R[st][0], R[st][1], R[st][2], V[st][0], V[st][1], V[st][2], bet,
+                             betsw, 0.0, m, ax, ay, az);
+            bool hit = false;
+            grain_fp64(P, 1, A.n_planets, R[st][0], R[st][1], R[st][2], 0.0, ax, ay, az, hit);
+            grain_fp64(P, A.n_planets, A.n_grav, R[st][0], R[st][1], R[st][2], A.eps2, ax, ay, az, hit);
+            gone = gone || hit;
+        }
+        if (st == 3) break;
+        const double a[3] = {ax, ay, az};
+        for (int c = 0; c < 3; ++c) {
+            R[st + 1][c] = __builtin_fma(V[st][c], hh[st], x0[c]);
+            V[st + 1][c] = __builtin_fma(a[c], hh[st], v0[c]);
+        }
+    }
+    const bool act = live && !gone;   // removed grains: identity map, nothing to reverse
+    double rbar[3] = {0, 0, 0}, vbar[3] = {0, 0, 0};
+    if (act)
+        for (int c = 0; c < 3; ++c) { rbar[c] = A.rb[c * na + t]; vbar[c] = A.vb[c * na + t]; }
+    const double w4[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+    double Vb[4][3], Ab[4][3];
+    for (int st = 0; st < 4; ++st)
+        for (int c = 0; c < 3; ++c) { Vb[st][c] = w4[st] * rbar[c]; Ab[st][c] = w4[st] * vbar[c]; }
+    double r_out[3] = {rbar[0], rbar[1], rbar[2]}, v_out[3] = {vbar[0], vbar[1], vbar[2]};
+    double betab = 0.0;
+    const double kap = (A.SWQ == 1) ? bet / A.light + betsw / A.vsw : bet / A.light;
+    const double dkap = 1.0 / A.light;
+    const size_t n = (size_t)A.n_grav;
+    for (int st = 3; st >= 0; --st) {
+        const double4* P = A.p64[st];
+        const double* Rs = R[st];
+        const double* Vs = V[st];
+        const double* Av = Ab[st];
+        double Rb[3] = {0, 0, 0};
+        double* wp = A.wpart + ((size_t)wave * 4 + st) * n * 4;
+        {   // star term
+            const double4 s0 = P[0];
+            const double m0 = s0.w;
+            const double d[3] = {Rs[0] - s0.x, Rs[1] - s0.y, Rs[2] - s0.z};
+            const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double Rr = __builtin_sqrt(R2), R3 = R2 * Rr, R4 = R2 * R2, R5 = R3 * R2, R6 = R4 * R2;
+            const double Gc = (1.0 - bet - betsw) * m0;
+            const double dv = d[0] * Vs[0] + d[1] * Vs[1] + d[2] * Vs[2];
+            const double dA = d[0] * Av[0] + d[1] * Av[1] + d[2] * Av[2];
+            const double vA = Vs[0] * Av[0] + Vs[1] * Av[1] + Vs[2] * Av[2];
+            double db[3], t2A = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double J0A = Av[c] / R3 - 3.0 * d[c] * dA / R5;
+                db[c] = -Gc * J0A - m0 * kap * ((Vs[c] * dA + dv * Av[c]) / R4 - 4.0 * dv * dA * d[c] / R6 -
+                                                 2.0 * d[c] * vA / R4);
+                Vb[st][c] += act ? -m0 * kap * (d[c] * dA / R4 + Av[c] / R2) : 0.0;
+                t2A += (dv * d[c] / R4 + Vs[c] / R2) * Av[c];
+            }
+            const double bb = m0 * dA / R3 - m0 * dkap * t2A;
+            const double mb = -(1.0 - bet - betsw) * dA / R3 - kap * t2A;
+            double c4[4];
+            for (int c = 0; c < 3; ++c) {
+                if (act) Rb[c] += db[c];
+                c4[c] = act ? -db[c] : 0.0;
+            }
+            c4[3] = act ? mb : 0.0;
+            if (act) betab += bb;
+            for (int c = 0; c < 4; ++c) {
+                const double s = wave_sum(c4[c]);
+                if (lane == 0) wp[c] = s;
+            }
+        }
+        for (int j = 1; j < A.n_grav; ++j) {   // massive pulls
+            const double4 pj = P[j];
+            const double q[3] = {pj.x - Rs[0], pj.y - Rs[1], pj.z - Rs[2]};
+            const double s2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + ((j >= A.n_planets) ? A.eps2 : 0.0);
+            const double inv = 1.0 / __builtin_sqrt(s2), inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            const double qA = q[0] * Av[0] + q[1] * Av[1] + q[2] * Av[2];
+            double c4[4];
+            for (int c = 0; c < 3; ++c) {
+                const double JA = Av[c] * inv3 - 3.0 * q[c] * qA * inv5;
+                c4[c] = act ? pj.w * JA : 0.0;
+                if (act) Rb[c] -= pj.w * JA;
+            }
+            c4[3] = act ? qA * inv3 : 0.0;
+            for (int c = 0; c < 4; ++c) {
+                const double s = wave_sum(c4[c]);
+                if (lane == 0) wp[(size_t)j * 4 + c] = s;
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            r_out[c] += Rb[c];
+            v_out[c] += Vb[st][c];
+            if (st > 0) {
+                Vb[st - 1][c] += hh[st - 1] * Rb[c];
+                Ab[st - 1][c] += hh[st - 1] * Vb[st][c];
+            }
+        }
+    }
+    if (act) {
+        for (int c = 0; c < 3; ++c) { A.rb[c * na + t] = r_out[c]; A.vb[c * na + t] = v_out[c]; }
+        A.g_beta[g] += betab;
+    }
+}
+
+// Sum the per-wave grain partials in wave order: Rbg[st][c][j], mug[j].
+__global__ void k_adj_reduce(AdjArgs A)
+{
+    const int item = blockIdx.x * kWG + threadIdx.x;   // (st, j)
+    const int n = A.n_grav;
+    if (item >= 4 * n) return;
+    const int st = item / n, j = item % n;
+    double s[4] = {0, 0, 0, 0};
+    for (int w = 0; w < A.nwaves; ++w) {
+        const double* p = A.wpart + (((size_t)w * 4 + st) * n + j) * 4;
+        for (int c = 0; c < 4; ++c) s[c] += p[c];
+    }
+    for (int c = 0; c < 3; ++c) A.Rbg[((size_t)st * 3 + c) * n + j] = s[c];
+    if (st == 0) {
+        double mt = s[3];
+        for (int s2 = 1; s2 < 4; ++s2) {
+            double acc = 0.0;
+            for (int w = 0; w < A.nwaves; ++w) acc += A.wpart[(((size_t)w * 4 + s2) * n + j) * 4 + 3];
+            mt += acc;
+        }
+        A.mug[j] = mt;
+    }
+}
+
+// Massive stage reverse (stage ST): combine VJP pair partials with the grain
+// pulls, update rbar, vbar and the previous stage's Vbar and Abar.
+template <int ST>
+__global__ void k_adj_mstage(AdjArgs A, int with_grains)
+{
+    const int i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= A.n_grav) return;
+    const size_t n = (size_t)A.n_grav, na = A.n_all;
+    const double h = A.dt;
+    const double hh[3] = {h / 2.0, h / 2.0, h};
+    double Rb[3] = {0, 0, 0}, mb = 0.0;
+    for (int y = 0; y < A.nslice; ++y) {
+        const double* p = A.part + (size_t)y * 4 * n;
+        Rb[0] += p[i]; Rb[1] += p[n + i]; Rb[2] += p[2 * n + i]; mb += p[3 * n + i];
+    }
+    if (with_grains) {
+        for (int c = 0; c < 3; ++c) Rb[c] += A.Rbg[((size_t)ST * 3 + c) * n + i];
+        if (ST == 3) mb += A.mug[i];
+    }
+    A.g_mu[i] += mb;
+    for (int c = 0; c < 3; ++c) {
+        const double Vb = A.Vbm[(ST * 3 + c) * n + i];
+        A.rb[c * na + i] += Rb[c];
+        A.vb[c * na + i] += Vb;
+        if (ST > 0) {
+            A.Vbm[((ST - 1) * 3 + c) * n + i] += hh[ST - 1] * Rb[c];
+            A.Abm[((ST - 1) * 3 + c) * n + i] += hh[ST - 1] * Vb;
+        }
+    }
+}
+
+// Copy of the full state (x, y, z, vx, vy, vz, N: 7 blocks of n_all).
+__global__ void k_copy_state(size_t n_all, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz, const double* N,
+                             double* dst)
+{
+    const size_t i = (size_t)blockIdx.x * kWG + threadIdx.x;
+    if (i >= n_all) return;
+    dst[i] = x[i]; dst[n_all + i] = y[i]; dst[2 * n_all + i] = z[i];
+    dst[3 * n_all + i] = vx[i]; dst[4 * n_all + i] = vy[i]; dst[5 * n_all + i] = vz[i];
+    dst[6 * n_all + i] = N[i];
+}
+
+__global__ void k_restore_state(size_t n_all, const double* src, double* x, double* y, double* z,
+                                double* vx, double* vy, double* vz, double* N)
+{
+    const size_t i = (size_t)blockIdx.x * kWG + threadIdx.x;
+    if (i >= n_all) return;
+    x[i] = src[i]; y[i] = src[n_all + i]; z[i] = src[2 * n_all + i];
+    vx[i] = src[3 * n_all + i]; vy[i] = src[4 * n_all + i]; vz[i] = src[5 * n_all + i];
+    N[i] = src[6 * n_all + i];
+}
+
+// ---------------------------------------------------------------------------
 // Potential of the massive set (diagnostics): phi_i = -sum_{j != i} mu_j / R_ij
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWG) void k_potential(const double4* __restrict__ pos, int n_grav,
@@ -998,6 +1325,117 @@ int ddyn_download_state(ddyn_ctx* c, double* x, double* y, double* z, double* vx
     const double* src[] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N};
     for (int a = 0; a < 7; ++a)
         if (dst[a]) DDYN_HIP(c, hipMemcpy(dst[a], src[a], bytes, hipMemcpyDeviceToHost));
+    return DDYN_OK;
+}
+
+namespace {
+
+struct DevBuf {
+    double* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, const double* lam_r,
+                 const double* lam_v, double* g_r, double* g_v, double* g_mu, double* g_beta)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!lam_r || !lam_v || !g_r || !g_v || !g_mu) return fail(c, DDYN_ERR_ARG, "adjoint arrays must not be NULL");
+    if (!(dt > 0.0) || nsteps < 0) return fail(c, DDYN_ERR_ARG, "need dt > 0 and nsteps >= 0");
+    if (c->prm.FLQ == 1) return fail(c, DDYN_ERR_ARG, "the adjoint does not support the Lorentz force (FLQ)");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav, nd = (size_t)c->n_dust;
+    const int64_t ck = (ckpt_every > 0 && ckpt_every < nsteps) ? ckpt_every : (nsteps > 0 ? nsteps : 1);
+    const int64_t nseg = nsteps > 0 ? (nsteps + ck - 1) / ck : 0;
+    const int nwaves = (int)((nd + kWG - 1) / kWG) * (kWG / 64);
+    // VJP source split over [0, n_grav)
+    const int tblocks = (int)((ng + kWG - 1) / kWG);
+    int vsl = std::max(1, std::min((kTargetWGs + tblocks - 1) / tblocks, (int)((ng + kTile - 1) / kTile)));
+    int vlen = (int)(((ng + vsl - 1) / vsl + kTile - 1) / kTile) * kTile;
+    vsl = (int)((ng + vlen - 1) / vlen);
+    DevBuf ckb, segb, finb, rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart;
+    if (dalloc(c, ckb.p, (size_t)std::max<int64_t>(nseg, 1) * 7 * na) || dalloc(c, segb.p, (size_t)ck * 7 * na) ||
+        dalloc(c, finb.p, 7 * na) || dalloc(c, rbb.p, 3 * na) || dalloc(c, vbb.p, 3 * na) ||
+        dalloc(c, Vbm.p, 12 * ng) || dalloc(c, Abm.p, 12 * ng) || dalloc(c, Rbg.p, 12 * ng) ||
+        dalloc(c, mug.p, ng) || dalloc(c, gmu.p, ng) || dalloc(c, gbeta.p, std::max<size_t>(nd, 1)) ||
+        dalloc(c, wpart.p, (size_t)std::max(nwaves, 1) * 16 * ng) || dalloc(c, vpart.p, (size_t)vsl * 4 * ng))
+        return DDYN_ERR_NOMEM;
+    const unsigned bn = (unsigned)((na + kWG - 1) / kWG), bg = (unsigned)((ng + kWG - 1) / kWG);
+    auto save = [&](double* dst) {
+        k_copy_state<<<bn, kWG, 0, c->stream>>>(na, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N, dst);
+    };
+    auto restore = [&](const double* src) {
+        k_restore_state<<<bn, kWG, 0, c->stream>>>(na, src, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N);
+        k_init_pos<<<bg, kWG, 0, c->stream>>>((int)ng, c->x, c->y, c->z, c->mu, c->p64[0], c->p32[0]);
+    };
+    // forward pass with checkpoints
+    for (int64_t k = 0; k < nsteps; ++k) {
+        if (k % ck == 0) save(ckb.p + (size_t)(k / ck) * 7 * na);
+        if ((rc = one_step(c, dt))) return rc;
+    }
+    save(finb.p);
+    DDYN_HIP(c, hipGetLastError());
+    DDYN_HIP(c, hipMemcpyAsync(rbb.p, lam_r, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    DDYN_HIP(c, hipMemcpyAsync(vbb.p, lam_v, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    DDYN_HIP(c, hipMemsetAsync(gmu.p, 0, ng * sizeof(double), c->stream));
+    DDYN_HIP(c, hipMemsetAsync(gbeta.p, 0, std::max<size_t>(nd, 1) * sizeof(double), c->stream));
+
+    AdjArgs A{};
+    A.n_grav = (int)ng; A.n_planets = c->prm.n_planets; A.n_dust = (int)nd; A.nslice = vsl;
+    A.nwaves = nwaves; A.n_all = na;
+    for (int s = 0; s < 4; ++s) A.p64[s] = c->p64[s];
+    A.x = c->x; A.y = c->y; A.z = c->z; A.vx = c->vx; A.vy = c->vy; A.vz = c->vz; A.mu = c->mu;
+    A.N = c->N; A.id = c->id;
+    A.beta_p = c->use_beta_p ? c->beta_p : nullptr;
+    A.betasw_p = (c->use_beta_p && c->have_betasw_p) ? c->betasw_p : nullptr;
+    A.beta_t = c->beta_t; A.betasw_t = c->betasw_t; A.n_bins = c->n_bins; A.SWQ = c->prm.SWQ;
+    A.dt = dt; A.eps2 = eps2_of(c); A.Rmin2 = c->prm.Rmin * c->prm.Rmin; A.vsw = c->prm.vsw;
+    A.light = c->prm.light > 0.0 ? c->prm.light : kLightAU;
+    A.rb = rbb.p; A.vb = vbb.p; A.Vbm = Vbm.p; A.Abm = Abm.p; A.Rbg = Rbg.p; A.mug = mug.p;
+    A.wpart = wpart.p; A.part = vpart.p; A.g_mu = gmu.p; A.g_beta = gbeta.p;
+    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
+
+    for (int64_t s = nseg - 1; s >= 0; --s) {
+        const int64_t k0 = s * ck, k1 = std::min(k0 + ck, nsteps);
+        restore(ckb.p + (size_t)s * 7 * na);
+        for (int64_t k = k0; k < k1; ++k) {
+            save(segb.p + (size_t)(k - k0) * 7 * na);
+            if (k + 1 < k1 && (rc = one_step(c, dt))) return rc;
+        }
+        for (int64_t k = k1 - 1; k >= k0; --k) {
+            restore(segb.p + (size_t)(k - k0) * 7 * na);
+            for (int st = 0; st < 3; ++st) {           // massive stage positions of step k
+                if ((rc = launch_pairs(c, st))) return rc;
+                if ((rc = launch_stage(c, st, stage_args(c, hs[st])))) return rc;
+            }
+            k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
+            if (nd) {
+                k_adj_dust<<<(unsigned)((nd + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
+                k_adj_reduce<<<(unsigned)((4 * ng + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
+            }
+            const int wg = nd ? 1 : 0;
+            for (int st = 3; st >= 0; --st) {
+                k_vjp_pairs<<<dim3(bg, (unsigned)vsl), kWG, 0, c->stream>>>(
+                    c->p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c), vpart.p);
+                switch (st) {
+                    case 3: k_adj_mstage<3><<<bg, kWG, 0, c->stream>>>(A, wg); break;
+                    case 2: k_adj_mstage<2><<<bg, kWG, 0, c->stream>>>(A, wg); break;
+                    case 1: k_adj_mstage<1><<<bg, kWG, 0, c->stream>>>(A, wg); break;
+                    default: k_adj_mstage<0><<<bg, kWG, 0, c->stream>>>(A, wg); break;
+                }
+            }
+            DDYN_HIP(c, hipGetLastError());
+        }
+    }
+    restore(finb.p);
+    DDYN_HIP(c, hipGetLastError());
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    DDYN_HIP(c, hipMemcpy(g_r, rbb.p, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    DDYN_HIP(c, hipMemcpy(g_v, vbb.p, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    DDYN_HIP(c, hipMemcpy(g_mu, gmu.p, ng * sizeof(double), hipMemcpyDeviceToHost));
+    if (g_beta && nd) DDYN_HIP(c, hipMemcpy(g_beta, gbeta.p, nd * sizeof(double), hipMemcpyDeviceToHost));
     return DDYN_OK;
 }
 

@@ -45,6 +45,7 @@ SIGNATURES = {
     "ddyn_accel": (ctypes.c_int, [_VP, _D, _D, _D, _I32]),
     "ddyn_download_state": (ctypes.c_int, [_VP] + [_D] * 7),
     "ddyn_energy": (ctypes.c_int, [_VP, _D]),
+    "ddyn_adjoint": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int64, ctypes.c_int64] + [_D] * 6),
     "ddyn_synchronize": (ctypes.c_int, [_VP]),
     "ddyn_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ddyn_read_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _I64, _D]),
@@ -184,6 +185,20 @@ class Engine:
         out = np.zeros(7)
         self._check(self.lib.ddyn_energy(self.ctx, _dp(out)), "ddyn_energy")
         return out
+
+    def adjoint(self, dt, nsteps, lam_r, lam_v, ckpt_every=0):
+        """Discrete adjoint from the uploaded state.  lam_r, lam_v: (n_all, 3)
+        dL/d(final r, v).  Returns (g_r, g_v) as (n_all, 3), g_mu (n_grav) and
+        g_beta (n_dust).  The device state ends at the state after nsteps."""
+        n = self.n_grav + self.n_dust
+        lr = np.ascontiguousarray(np.asarray(lam_r, dtype=np.float64).T).reshape(3 * n)
+        lv = np.ascontiguousarray(np.asarray(lam_v, dtype=np.float64).T).reshape(3 * n)
+        g_r, g_v = np.empty(3 * n), np.empty(3 * n)
+        g_mu = np.empty(self.n_grav)
+        g_beta = np.zeros(max(self.n_dust, 1))
+        self._check(self.lib.ddyn_adjoint(self.ctx, float(dt), int(nsteps), int(ckpt_every), _dp(lr), _dp(lv),
+                                          _dp(g_r), _dp(g_v), _dp(g_mu), _dp(g_beta)), "ddyn_adjoint")
+        return (g_r.reshape(3, n).T.copy(), g_v.reshape(3, n).T.copy(), g_mu, g_beta[: self.n_dust])
 
     def set_timing(self, enable=True):
         self._check(self.lib.ddyn_set_timing(self.ctx, 1 if enable else 0), "ddyn_set_timing")

@@ -115,6 +115,24 @@ int ddyn_download_state(ddyn_ctx *ctx, double *x, double *y, double *z,
  * No reference counterpart. */
 int ddyn_energy(ddyn_ctx *ctx, double *out7);
 
+/* Discrete adjoint of nsteps RK4 steps of size dt from the current state.
+ * No reference counterpart: the reference has no differentiation.
+ * Runs the forward pass (the context state ends at the state after nsteps, as
+ * ddyn_step would leave it) keeping a checkpoint every ckpt_every steps
+ * (<= 0: one checkpoint), then reverses every step, recomputing each segment
+ * from its checkpoint.  Arrays are SoA blocks: lam_r = [x | y | z] of n_all
+ * each = dL/d(final positions), lam_v likewise for velocities.  Outputs:
+ * g_r, g_v (3*n_all) = dL/d(initial positions, velocities); g_mu (n_grav) =
+ * dL/d mu of every massive body (direct force dependence); g_beta (n_dust,
+ * may be NULL) = dL/d beta per grain (meaningful with ddyn_upload_grain_beta).
+ * Grains removed during the run are treated as frozen.  The Lorentz force
+ * (FLQ) is not supported.  Stage forces are recomputed in the context's
+ * precision for massive bodies and in FP64 for grains; the Jacobian products
+ * are FP64. */
+int ddyn_adjoint(ddyn_ctx *ctx, double dt, int64_t nsteps, int64_t ckpt_every,
+                 const double *lam_r, const double *lam_v, double *g_r, double *g_v,
+                 double *g_mu, double *g_beta);
+
 /* Wait for all work on the context stream. */
 int ddyn_synchronize(ddyn_ctx *ctx);
 

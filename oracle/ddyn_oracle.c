@@ -327,6 +327,224 @@ int orc_step(const orc_params *p, orc_state *s, double dt, int64_t nsteps)
 }
 
 /* ------------------------------------------------------------------------- */
+/* Discrete adjoint of the RK4 step (no reference counterpart)               */
+/* ------------------------------------------------------------------------- */
+/* The step map is y -> y' with stages R1 = r, V1 = v; R2 = r + h/2 V1,
+ * V2 = v + h/2 A1; R3 = r + h/2 V2, V3 = v + h/2 A2; R4 = r + h V3,
+ * V4 = v + h A3; r' = r + h/6 (V1 + 2V2 + 2V3 + V4), v' = v + h/6 (A1 + ...),
+ * A_s = f(R_s, V_s).  Its reverse is written out stage by stage; the force
+ * Jacobians are, for K(d) = d / s^3 with s^2 = |d|^2 + eps^2,
+ *   dK/dd = J(d) = (I s^2 - 3 d d^T) / s^5   (symmetric, J(-d) = J(d)).
+ * Massive bodies: a_i = sum_j mu_j K(R_j - R_i), so
+ *   Rbar_k += sum_{j != k} J(R_j - R_k) (mu_k Abar_j - mu_j Abar_k),
+ *   mubar_k += sum_{i != k} Abar_i . K(R_k - R_i).
+ * Grains: the star term (gravity, radiation pressure, PR + wind drag) is
+ * differentiated in closed form; the Lorentz force is not supported here. */
+
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Rbar[k] += ..., mubar[k] += ... for all massive k at stage positions R. */
+static void massive_vjp(const orc_params *p, int64_t n, const double *Rx, const double *Ry,
+                        const double *Rz, const double *mu, const double *Ab /* 3n */,
+                        double *Rb /* 3n */, double *mub)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t k_swarm = (k >= p->n_planets);
+        double acc[3] = {0, 0, 0}, am = 0.0;
+        const double *Ak = &Ab[3 * k];
+        for (int64_t j = 0; j < n; j++) {
+            if (j == k) continue;
+            double d[3] = {Rx[j] - Rx[k], Ry[j] - Ry[k], Rz[j] - Rz[k]};
+            double s2 = dot3(d, d) + pair_eps2(p, k_swarm, j);
+            if (s2 == 0.0) continue;
+            double inv = 1.0 / sqrt(s2), inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            const double *Aj = &Ab[3 * j];
+            double w[3] = {mu[k] * Aj[0] - mu[j] * Ak[0], mu[k] * Aj[1] - mu[j] * Ak[1],
+                           mu[k] * Aj[2] - mu[j] * Ak[2]};
+            double dw = dot3(d, w);
+            for (int c = 0; c < 3; c++) acc[c] += w[c] * inv3 - 3.0 * d[c] * dw * inv5;
+            am -= dot3(Aj, d) * inv3;
+        }
+        for (int c = 0; c < 3; c++) Rb[3 * k + c] += acc[c];
+        mub[k] += am;
+    }
+}
+
+/* Grain state at the four stages of one step (forward, as grain_step). */
+typedef struct { double R[4][3], V[4][3]; int gone; } grain_stages;
+
+static void grain_forward_stages(const orc_params *p, const orc_state *s, const massive_ws *w,
+                                 int64_t t, double dt, double bet, double betsw, grain_stages *G)
+{
+    const double dt2 = dt / 2.0;
+    const double x0[3] = {s->x[t], s->y[t], s->z[t]}, v0[3] = {s->vx[t], s->vy[t], s->vz[t]};
+    const double hh[3] = {dt2, dt2, dt};
+    double A[3];
+    G->gone = 0;
+    for (int c = 0; c < 3; c++) { G->R[0][c] = x0[c]; G->V[0][c] = v0[c]; }
+    for (int st = 0; st < 4; st++) {
+        if (grain_accel(p, s->n_grav, w->kx[st], w->ky[st], w->kz[st], s->mu, G->R[st][0], G->R[st][1],
+                        G->R[st][2], G->V[st][0], G->V[st][1], G->V[st][2], bet, betsw, 0.0,
+                        s->mu[t], &A[0], &A[1], &A[2])) { G->gone = 1; return; }
+        if (st == 3) break;
+        for (int c = 0; c < 3; c++) {
+            G->R[st + 1][c] = fma(G->V[st][c], hh[st], x0[c]);
+            G->V[st + 1][c] = fma(A[c], hh[st], v0[c]);
+        }
+    }
+}
+
+/* Reverse of one grain's step.  rb, vb: in = dL/d(state after), out = dL/d(state
+ * before).  Accumulates the grain's pull on the massive stage positions into
+ * Rbm[st][3*j] and on the masses into mub[j]; betab gets dL/dbeta. */
+static void grain_reverse(const orc_params *p, const orc_state *s, const massive_ws *w, int64_t t,
+                          double dt, double bet, double betsw, const grain_stages *G,
+                          double *rb, double *vb, double *betab, double *Rbm[4], double *mub)
+{
+    const double h = dt;
+    const double wts[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0};
+    double Vb[4][3], Ab[4][3];
+    for (int st = 0; st < 4; st++)
+        for (int c = 0; c < 3; c++) { Vb[st][c] = wts[st] * rb[c]; Ab[st][c] = wts[st] * vb[c]; }
+    double r_bar[3] = {rb[0], rb[1], rb[2]}, v_bar[3] = {vb[0], vb[1], vb[2]};
+    const double m0 = s->mu[0];
+    const double kap = (p->SWQ == 1) ? bet / p->light + betsw / p->vsw : bet / p->light;
+    const double dkap = 1.0 / p->light;            /* d kappa / d beta */
+    const double e2 = p->eps * p->eps;
+    const int64_t ng = s->n_grav;
+    const double hh[3] = {h / 2.0, h / 2.0, h};
+    for (int st = 3; st >= 0; st--) {
+        const double *Rs = G->R[st], *Vs = G->V[st], *A = Ab[st];
+        double Rb[3] = {0, 0, 0};
+        /* star term */
+        double d[3] = {Rs[0] - w->kx[st][0], Rs[1] - w->ky[st][0], Rs[2] - w->kz[st][0]};
+        double R2 = dot3(d, d), R = sqrt(R2), R3 = R2 * R, R4 = R2 * R2, R5 = R3 * R2, R6 = R4 * R2;
+        double Gc = (1.0 - bet - betsw) * m0;
+        double dv = dot3(d, Vs), dA = dot3(d, A), vA = dot3(Vs, A);
+        double db[3];
+        for (int c = 0; c < 3; c++) {
+            double J0A = A[c] / R3 - 3.0 * d[c] * dA / R5;
+            db[c] = -Gc * J0A - m0 * kap * ((Vs[c] * dA + dv * A[c]) / R4 - 4.0 * dv * dA * d[c] / R6
+                                              - 2.0 * d[c] * vA / R4);
+            Vb[st][c] += -m0 * kap * (d[c] * dA / R4 + A[c] / R2);
+        }
+        double term2[3];
+        for (int c = 0; c < 3; c++) term2[c] = dv * d[c] / R4 + Vs[c] / R2;
+        *betab += m0 * dA / R3 - m0 * dkap * dot3(term2, A);
+        mub[0] += -(1.0 - bet - betsw) * dA / R3 - kap * dot3(term2, A);
+        for (int c = 0; c < 3; c++) { Rb[c] += db[c]; Rbm[st][c] -= db[c]; }
+        /* massive pulls, j >= 1: a += mu_j K(R_j - R_t) */
+        for (int64_t j = 1; j < ng; j++) {
+            double q[3] = {w->kx[st][j] - Rs[0], w->ky[st][j] - Rs[1], w->kz[st][j] - Rs[2]};
+            double s2 = dot3(q, q) + ((j >= p->n_planets) ? e2 : 0.0);
+            double inv = 1.0 / sqrt(s2), inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            double qA = dot3(q, A);
+            for (int c = 0; c < 3; c++) {
+                double JA = A[c] * inv3 - 3.0 * q[c] * qA * inv5;
+                Rbm[st][3 * j + c] += s->mu[j] * JA;
+                Rb[c] -= s->mu[j] * JA;
+            }
+            mub[j] += qA * inv3;
+        }
+        /* chain through the stage definitions */
+        for (int c = 0; c < 3; c++) {
+            r_bar[c] += Rb[c];
+            if (st == 0) { v_bar[c] += Vb[0][c]; continue; }
+            v_bar[c] += Vb[st][c];
+            Vb[st - 1][c] += hh[st - 1] * Rb[c];
+            Ab[st - 1][c] += hh[st - 1] * Vb[st][c];
+        }
+    }
+    for (int c = 0; c < 3; c++) { rb[c] = r_bar[c]; vb[c] = v_bar[c]; }
+}
+
+int orc_adjoint(const orc_params *p, const orc_state *s0, double dt, int64_t nsteps,
+                const double *lam_r, const double *lam_v, double *g_r, double *g_v,
+                double *g_mu, double *g_beta)
+{
+    if (p->FLQ == 1) return -2;
+    const int64_t ng = s0->n_grav, nd = s0->n_dust, na = ng + nd;
+    const size_t per = (size_t)na * 7;
+    double *hist = (double *)malloc(per * (size_t)(nsteps + 1) * sizeof(double));
+    double *Rbm[4];
+    double *mub = (double *)calloc((size_t)ng, sizeof(double));
+    double *Abm = (double *)calloc(3 * 4 * (size_t)ng, sizeof(double));
+    double *Vbm = (double *)calloc(3 * 4 * (size_t)ng, sizeof(double));
+    for (int st = 0; st < 4; st++) Rbm[st] = (double *)calloc(3 * (size_t)ng, sizeof(double));
+    massive_ws w;
+    memset(&w, 0, sizeof w);
+    if (!hist || !mub || !Abm || !Vbm || !Rbm[0] || !Rbm[1] || !Rbm[2] || !Rbm[3] || ws_alloc(&w, ng)) {
+        free(hist); free(mub); free(Abm); free(Vbm);
+        for (int st = 0; st < 4; st++) free(Rbm[st]);
+        ws_free(&w);
+        return -1;
+    }
+    /* forward, keeping every step's state (x,y,z,vx,vy,vz,N) */
+    orc_state s = *s0;
+    double *cur = (double *)malloc(per * sizeof(double));
+    double *arr[7];
+    for (int a = 0; a < 7; a++) arr[a] = cur + (size_t)a * na;
+    const double *src0[7] = {s0->x, s0->y, s0->z, s0->vx, s0->vy, s0->vz, s0->N};
+    for (int a = 0; a < 7; a++) memcpy(arr[a], src0[a], (size_t)na * sizeof(double));
+    s.x = arr[0]; s.y = arr[1]; s.z = arr[2]; s.vx = arr[3]; s.vy = arr[4]; s.vz = arr[5]; s.N = arr[6];
+    memcpy(hist, cur, per * sizeof(double));
+    for (int64_t k = 0; k < nsteps; k++) {
+        orc_step(p, &s, dt, 1);
+        memcpy(hist + per * (size_t)(k + 1), cur, per * sizeof(double));
+    }
+    /* adjoint state */
+    double *rb = g_r, *vb = g_v;
+    memcpy(rb, lam_r, 3 * (size_t)na * sizeof(double));
+    memcpy(vb, lam_v, 3 * (size_t)na * sizeof(double));
+    memset(g_mu, 0, (size_t)ng * sizeof(double));
+    if (g_beta) memset(g_beta, 0, (size_t)(nd > 0 ? nd : 0) * sizeof(double));
+    const double h = dt, wts[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0}, hh[3] = {h / 2.0, h / 2.0, h};
+    for (int64_t k = nsteps - 1; k >= 0; k--) {
+        memcpy(cur, hist + per * (size_t)k, per * sizeof(double));
+        massive_stages(p, &s, &w, dt);     /* stage positions of step k */
+        for (int st = 0; st < 4; st++) memset(Rbm[st], 0, 3 * (size_t)ng * sizeof(double));
+        memset(mub, 0, (size_t)ng * sizeof(double));
+        /* grains (serial: they scatter into the massive adjoints) */
+        for (int64_t g = 0; g < nd; g++) {
+            int64_t t = ng + g;
+            if (s.N[t] == 0.0) continue;
+            double bet, betsw, qe;
+            grain_coeffs(&s, t, &bet, &betsw, &qe);
+            grain_stages G;
+            grain_forward_stages(p, &s, &w, t, dt, bet, betsw, &G);
+            if (G.gone) continue;                        /* removed this step: identity map */
+            double betab = 0.0;
+            grain_reverse(p, &s, &w, t, dt, bet, betsw, &G, &rb[3 * t], &vb[3 * t], &betab, Rbm, mub);
+            if (g_beta) g_beta[g] += betab;
+        }
+        /* massive bodies: Vbar_s = w_s rbar', Abar_s = w_s vbar' (before any update) */
+        for (int st = 0; st < 4; st++)
+            for (int64_t i = 0; i < 3 * ng; i++) {
+                Vbm[st * 3 * ng + i] = wts[st] * rb[i];
+                Abm[st * 3 * ng + i] = wts[st] * vb[i];
+            }
+        for (int st = 3; st >= 0; st--) {
+            massive_vjp(p, ng, w.kx[st], w.ky[st], w.kz[st], s.mu, &Abm[st * 3 * ng], Rbm[st], mub);
+            for (int64_t i = 0; i < 3 * ng; i++) {
+                const double Rb = Rbm[st][i], Vb = Vbm[st * 3 * ng + i];
+                rb[i] += Rb;                       /* R_s = r + ... */
+                vb[i] += Vb;                       /* V_s = v + ... (V_1 = v) */
+                if (st > 0) {
+                    Vbm[(st - 1) * 3 * ng + i] += hh[st - 1] * Rb;   /* R_s = r + h' V_{s-1} */
+                    Abm[(st - 1) * 3 * ng + i] += hh[st - 1] * Vb;   /* V_s = v + h' A_{s-1} */
+                }
+            }
+        }
+        for (int64_t j = 0; j < ng; j++) g_mu[j] += mub[j];
+    }
+    free(cur); free(hist); free(mub); free(Abm); free(Vbm);
+    for (int st = 0; st < 4; st++) free(Rbm[st]);
+    ws_free(&w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Invariants of the massive set (units: mu = G*M, so these are G*E, G*L, G*P) */
 /* ------------------------------------------------------------------------- */
 

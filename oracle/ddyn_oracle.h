@@ -45,6 +45,13 @@ void orc_massive_accel_targets(const orc_params *p, int64_t n_grav,
 int  orc_dust_accel(const orc_params *p, const orc_state *s, double *ax, double *ay, double *az,
                     int32_t *removed);
 int  orc_step(const orc_params *p, orc_state *s, double dt, int64_t nsteps);
+/* Discrete adjoint of nsteps RK4 steps from the state s (not modified).
+ * lam_r, lam_v: dL/d(final r, v), 3*n_all interleaved (x,y,z per particle).
+ * Outputs: g_r, g_v (3*n_all) = dL/d(initial r, v); g_mu (n_grav) = dL/dmu;
+ * g_beta (n_dust, may be NULL) = dL/dbeta.  Returns -2 if FLQ is on. */
+int  orc_adjoint(const orc_params *p, const orc_state *s, double dt, int64_t nsteps,
+                 const double *lam_r, const double *lam_v, double *g_r, double *g_v,
+                 double *g_mu, double *g_beta);
 void orc_energy(const orc_params *p, int64_t n, const double *x, const double *y, const double *z,
                 const double *vx, const double *vy, const double *vz, const double *mu,
                 double *out7);

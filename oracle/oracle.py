@@ -58,6 +58,9 @@ def lib():
         L.orc_step.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcState), ctypes.c_double,
                                ctypes.c_int64]
         L.orc_step.restype = ctypes.c_int
+        L.orc_adjoint.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcState), ctypes.c_double,
+                                  ctypes.c_int64] + [_D] * 6
+        L.orc_adjoint.restype = ctypes.c_int
         L.orc_energy.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int64] + [_D] * 8
         L.orc_energy.restype = None
         L.orc_coe2rv.argtypes = [ctypes.c_double] * 9 + [_D]
@@ -142,6 +145,22 @@ def step(p, st, dt, nsteps):
     if rc != 0:
         raise MemoryError("oracle workspace allocation failed")
     return st
+
+
+def adjoint(p, st, dt, nsteps, lam_r, lam_v):
+    """Discrete adjoint of nsteps steps from st (unchanged).  lam_r, lam_v:
+    (n_all, 3) = dL/d(final r, v).  Returns (g_r, g_v, g_mu, g_beta)."""
+    na = st.n_all
+    lr = np.ascontiguousarray(lam_r, dtype=np.float64).reshape(na * 3)
+    lv = np.ascontiguousarray(lam_v, dtype=np.float64).reshape(na * 3)
+    g_r, g_v = np.zeros(na * 3), np.zeros(na * 3)
+    g_mu, g_beta = np.zeros(st.n_grav), np.zeros(max(st.n_dust, 1))
+    b = _Bound(st)
+    rc = lib().orc_adjoint(ctypes.byref(params(p)), ctypes.byref(b.c), float(dt), int(nsteps),
+                           _p(lr), _p(lv), _p(g_r), _p(g_v), _p(g_mu), _p(g_beta))
+    if rc != 0:
+        raise RuntimeError(f"orc_adjoint failed ({rc})")
+    return g_r.reshape(na, 3), g_v.reshape(na, 3), g_mu, g_beta[: st.n_dust]
 
 
 def energy(p, st):
