@@ -1,33 +1,63 @@
-"""In-tree build of libddyn_hip.so for gfx950 with hipcc.
+"""In-tree build of the native parts.
 
-Flags: -O3, C++17, --offload-arch=gfx950, -ffp-contract=off (every fused
-multiply-add in the kernels is written out explicitly, so rounding is fixed),
-and the packed-FP32 target feature switched off.
+* libddyn_hip.so  HIP kernels + C ABI (include/ddyn.h), hipcc for gfx950 with
+  -O3, C++17, -ffp-contract=off (every fused multiply-add in the kernels is
+  written out explicitly, so rounding is fixed) and the packed-FP32 target
+  feature switched off.
+* libddyn_host.so .param reader, time-step rule, data-file writer/reader
+  (include/ddyn_host.h), host C++ only.
+* ddyn            command-line driver linking both.
 """
 import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = [os.path.join(HERE, "csrc", "ddyn_hip.hip")]
+CXX = os.environ.get("CXX", "g++")
 LIB = os.path.join(HERE, "libddyn_hip.so")
+HOST_LIB = os.path.join(HERE, "libddyn_host.so")
+CLI = os.path.join(HERE, "ddyn")
 HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-I" + os.path.join(ROOT, "include")]
+             "-I" + INC]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I" + INC]
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
 
 
 def build_hip(verbose=False):
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + SOURCES + ["-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "ddyn_hip.hip"), "-o", tmp], "hipcc")
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
     return LIB
 
 
+def build_host(verbose=False):
+    tmp = HOST_LIB + ".tmp"
+    _run([CXX] + HOST_FLAGS + ["-shared", os.path.join(CSRC, "ddyn_host.cpp"), "-o", tmp], "host library")
+    os.replace(tmp, HOST_LIB)
+    tmp = CLI + ".tmp"
+    _run([CXX] + HOST_FLAGS + [os.path.join(CSRC, "ddyn_main.cpp"), "-o", tmp, "-L" + HERE,
+                               "-lddyn_hip", "-lddyn_host", "-Wl,-rpath,$ORIGIN"], "ddyn driver")
+    os.replace(tmp, CLI)
+    if verbose:
+        print("built", HOST_LIB, "and", CLI)
+    return HOST_LIB
+
+
+def build_all(verbose=False):
+    build_hip(verbose)
+    build_host(verbose)
+
+
 if __name__ == "__main__":
-    build_hip(verbose=True)
+    build_all(verbose=True)

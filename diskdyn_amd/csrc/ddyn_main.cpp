@@ -1,0 +1,205 @@
+// ddyn_main.cpp -- command-line driver of the dynamical hot path on one GPU.
+//
+//   ddyn -i <file.param> [options]   start from a .param file (star + planets)
+//   ddyn -m <file.dat>   [options]   continue from a data file (every particle)
+//
+// It replaces the dynamics of the reference's main + Grid::evolve
+// (src/ddyn.cu:37-111, src/grid.cu:344-439): the same time step (det_step),
+// the same clipping of steps to the data/FITS/SED output times, and data files
+// <stub>_<time>.dat written at the data cadence and at the end.  Imaging and
+// SEDs are not produced; their output times still clip the step so the
+// sequence of steps matches the reference.
+//
+// Options: --fp32 (FP32-force form), --eps <AU> (softening of swarm pairs),
+//          --rmin <AU> (sublimation radius), --device <n>, --quiet.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ddyn.h"
+#include "ddyn_host.h"
+
+namespace {
+
+constexpr const char* kVersion = "1.01";
+constexpr double kAU = 1.49598e11;
+constexpr double kYear = 3.1556926e7;
+
+void usage()
+{
+    std::fprintf(stderr,
+                 "Usage: ddyn -i <init.param> | -m <model.dat> [--fp32] [--eps AU] [--rmin AU] "
+                 "[--device n] [--quiet]\n");
+}
+
+struct State {
+    int64_t n_grav = 0, n_dust = 0;
+    std::vector<double> x, y, z, vx, vy, vz, N, mu, beta;
+    std::vector<int32_t> id;
+    void resize(int64_t n)
+    {
+        for (auto* v : {&x, &y, &z, &vx, &vy, &vz, &N, &mu, &beta}) v->assign((size_t)n, 0.0);
+        id.assign((size_t)n, -1);
+    }
+};
+
+int check(ddyn_ctx* ctx, int rc, const char* what)
+{
+    if (rc) std::fprintf(stderr, "ddyn: %s failed (%d): %s\n", what, rc, ddyn_last_error(ctx));
+    return rc;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    const char* param = nullptr;
+    const char* model = nullptr;
+    int precision = DDYN_FP64, device = 0, quiet = 0;
+    double eps = 0.0, rmin = 0.0;
+    for (int a = 1; a < argc; ++a) {
+        const std::string s = argv[a];
+        if (s == "-i" && a + 1 < argc) param = argv[++a];
+        else if (s == "-m" && a + 1 < argc) model = argv[++a];
+        else if (s == "--fp32") precision = DDYN_FP32_FORCE;
+        else if (s == "--eps" && a + 1 < argc) eps = std::atof(argv[++a]);
+        else if (s == "--rmin" && a + 1 < argc) rmin = std::atof(argv[++a]);
+        else if (s == "--device" && a + 1 < argc) device = std::atoi(argv[++a]);
+        else if (s == "--quiet") quiet = 1;
+        else { usage(); return s == "-h" ? 0 : 1; }
+    }
+    if (!param == !model) { usage(); return 1; }
+
+    static ddyn_config cfg;
+    ddyn_run_info run{};
+    State st;
+    int rc;
+    bool param_mode = param != nullptr;
+    if (param_mode) {
+        if ((rc = ddyn_read_param(param, &cfg))) {
+            std::fprintf(stderr, "ddyn: %s: %s\n", param, ddyn_host_error_text(rc));
+            return 1;
+        }
+        for (int k = 0; k < cfg.n_disks; ++k)
+            if (cfg.disks[k].Ngrav || cfg.disks[k].Nsmall) {
+                std::fprintf(stderr, "ddyn: disk particle generation is not part of this build; "
+                                     "start from a data file with -m\n");
+                return 2;
+            }
+        for (int k = 0; k < cfg.n_blobs; ++k)
+            if (cfg.blobs[k].Ngrav || cfg.blobs[k].Nsmall) {
+                std::fprintf(stderr, "ddyn: blob particle generation is not part of this build; "
+                                     "start from a data file with -m\n");
+                return 2;
+            }
+        st.n_grav = cfg.n_planets;
+        st.resize(st.n_grav);
+        for (int j = 0; j < cfg.n_planets; ++j) {
+            st.x[j] = cfg.px[j]; st.y[j] = cfg.py[j]; st.z[j] = cfg.pz[j];
+            st.vx[j] = cfg.pvx[j]; st.vy[j] = cfg.pvy[j]; st.vz[j] = cfg.pvz[j];
+            st.mu[j] = cfg.pm[j]; st.N[j] = 1.0;
+        }
+        run.n_mass = 0; run.n_dust = 0; run.n_all = st.n_grav;
+    } else {
+        if ((rc = ddyn_read_datafile(model, kVersion, &cfg, &run, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                     nullptr, nullptr, nullptr, nullptr, nullptr))) {
+            std::fprintf(stderr, "ddyn: %s: %s\n", model, ddyn_host_error_text(rc));
+            return 1;
+        }
+        st.resize(run.n_all);
+        if ((rc = ddyn_read_datafile(model, kVersion, &cfg, &run, st.x.data(), st.y.data(), st.z.data(),
+                                     st.vx.data(), st.vy.data(), st.vz.data(), st.N.data(), st.mu.data(),
+                                     st.id.data(), st.beta.data()))) {
+            std::fprintf(stderr, "ddyn: %s: %s\n", model, ddyn_host_error_text(rc));
+            return 1;
+        }
+        st.n_dust = run.n_dust;
+        st.n_grav = run.n_all - run.n_dust;
+    }
+    run.Rmax = 0.0;
+    for (int k = 0; k < cfg.n_disks; ++k) run.Rmax = std::max(run.Rmax, cfg.disks[k].Rout);
+    for (int k = 0; k < cfg.n_blobs; ++k)
+        run.Rmax = std::max(run.Rmax, cfg.blobs[k].a * (1.0 + cfg.blobs[k].ecc) + 3.0 * cfg.blobs[k].sigma);
+    if (run.Rmax <= 0.0) run.Rmax = 1e300;
+
+    ddyn_ctx* ctx = nullptr;
+    if ((rc = ddyn_create(&ctx, device))) {
+        std::fprintf(stderr, "ddyn: no usable GPU (device %d, code %d)\n", device, rc);
+        return 1;
+    }
+    ddyn_params prm{};
+    prm.n_planets = cfg.n_planets;
+    prm.SWQ = cfg.SWQ;
+    prm.FLQ = cfg.FLQ;
+    prm.precision = precision;
+    prm.eps = eps;
+    prm.Rmin = rmin;
+    prm.vsw = cfg.vsw * kYear / kAU;
+    prm.Wz = cfg.Wz;
+    prm.Bstar = cfg.Bstar;
+    prm.Rstar = cfg.rstar;
+    prm.light = 0.0;
+    prm.fp64_radius = 0.0;
+    if (check(ctx, ddyn_set_params(ctx, &prm), "set_params") ||
+        check(ctx, ddyn_upload_state(ctx, st.n_grav, st.n_dust, st.x.data(), st.y.data(), st.z.data(), st.vx.data(),
+                                     st.vy.data(), st.vz.data(), st.mu.data(), st.N.data(), st.id.data()),
+              "upload_state") ||
+        check(ctx, ddyn_set_params(ctx, &prm), "set_params")) {
+        ddyn_destroy(ctx);
+        return 1;
+    }
+    if (st.n_dust) {   // beta per grain from the data file's beta column; no wind or charge tables
+        std::vector<double> b(st.beta.begin() + st.n_grav, st.beta.end());
+        if (check(ctx, ddyn_upload_grain_beta(ctx, b.data(), nullptr), "upload_grain_beta")) {
+            ddyn_destroy(ctx);
+            return 1;
+        }
+    }
+    auto write = [&](double time) -> int {
+        if (check(ctx, ddyn_download_state(ctx, st.x.data(), st.y.data(), st.z.data(), st.vx.data(), st.vy.data(),
+                                           st.vz.data(), st.N.data()), "download_state"))
+            return 1;
+        char name[600];
+        std::snprintf(name, sizeof name, "%s_%.2f.dat", cfg.name_stub[0] ? cfg.name_stub : "ddyn", time);
+        const double* bp = st.n_dust ? st.beta.data() + st.n_grav : nullptr;
+        int r = ddyn_write_datafile(name, kVersion, &cfg, &run, st.x.data(), st.y.data(), st.z.data(),
+                                    st.vx.data(), st.vy.data(), st.vz.data(), st.N.data(), st.mu.data(),
+                                    st.id.data(), nullptr, 0, bp);
+        if (!quiet) std::printf("ddyn: wrote %s\n", name);
+        return r;
+    };
+
+    // Grid::evolve (src/grid.cu:344-439)
+    const double dt = ddyn_det_step(&cfg, st.n_grav, st.x.data(), st.y.data(), st.z.data(), st.mu.data());
+    run.dt = dt;
+    double dttmp = dt;
+    if (!quiet) std::printf("ddyn: dt = %.8e yr, t_end = %.3f yr\n", dt, cfg.t_end);
+    if (param_mode && write(run.time)) { ddyn_destroy(ctx); return 1; }
+    auto clip = [&]() {
+        if (run.time + dttmp > run.tlastd + cfg.t_write) dttmp = cfg.t_write + run.tlastd - run.time;
+        if (run.time + dttmp > run.tlastf + cfg.t_fits) dttmp = cfg.t_fits + run.tlastf - run.time;
+        if (run.time + dttmp > run.tlasts + cfg.t_sed) dttmp = cfg.t_sed + run.tlasts - run.time;
+        if (run.time + dttmp > cfg.t_end) dttmp = cfg.t_end - run.time;
+    };
+    clip();
+    int64_t steps = 0;
+    do {
+        if (check(ctx, ddyn_step(ctx, dttmp, 1), "step")) { ddyn_destroy(ctx); return 1; }
+        ++steps;
+        run.time += dttmp;
+        if (run.time >= run.tlasts + cfg.t_sed || run.time >= cfg.t_end) { run.tlasts = run.time; dttmp = dt; }
+        if (run.time >= run.tlastf + cfg.t_fits || run.time >= cfg.t_end) { run.tlastf = run.time; dttmp = dt; }
+        if (run.time >= run.tlastd + cfg.t_write || run.time >= cfg.t_end) {
+            run.tlastd = run.time;
+            if (write(run.time)) { ddyn_destroy(ctx); return 1; }
+            dttmp = dt;
+        }
+        clip();
+    } while (run.time < cfg.t_end);
+    if (!quiet) std::printf("ddyn: %lld steps to t = %.6f yr\n", (long long)steps, run.time);
+    ddyn_destroy(ctx);
+    return 0;
+}

@@ -1,0 +1,149 @@
+"""ctypes binding of libddyn_host.so (include/ddyn_host.h): .param reader,
+det_step and the data-file writer/reader.  Host code only (no GPU)."""
+import ctypes
+import os
+
+import numpy as np
+
+from .system import DiskState, Params, make_state
+from .units import AU, YEAR
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB_PATH = os.path.join(_HERE, "libddyn_host.so")
+VERSION = "1.01"
+
+MAXP, MAXD, MAXB, MAXW, STR = 64, 16, 16, 64, 256
+_D = ctypes.c_double
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+
+class Disk(ctypes.Structure):
+    _fields_ = [(n, _D) for n in ("Rin", "Rout", "dhdr", "sigmae", "amax", "amin", "eta", "Mtot", "mmax",
+                                   "mmin", "gammas", "dmax", "dmmax", "C")] + [("Ngrav", _I64), ("Nsmall", _I64)]
+
+
+class Blob(ctypes.Structure):
+    _fields_ = [(n, _D) for n in ("a", "ecc", "inc", "w", "o", "nu", "sigma", "amax", "amin", "eta", "Mtot",
+                                   "mmax", "mmin", "vkepp", "dmax", "dmmax", "C")] + \
+               [("Ngrav", _I64), ("Nsmall", _I64), ("collcloud", _I32), ("pad_", _I32)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("verb", _I32), ("gpuid", _I32), ("gl", _I32), ("n_step", _I32)] + \
+               [(n, _D) for n in ("t_gl", "t_write", "t_fits", "t_sed", "t_end")] + \
+               [("sptype", ctypes.c_char * STR)] + \
+               [(n, _D) for n in ("teff", "rstar", "Bstar", "Wz", "mloss", "vsw", "dist", "fov", "PA", "inc",
+                                   "longitude")] + \
+               [("xsize", _I32), ("ysize", _I32), ("nwavs", _I32), ("radQ", _I32), ("wavs", _D * MAXW),
+                ("specfile", ctypes.c_char * STR), ("SWQ", _I32), ("FLQ", _I32), ("n_planets", _I32),
+                ("n_disks", _I32)] + \
+               [(n, _D * MAXP) for n in ("px", "py", "pz", "pvx", "pvy", "pvz", "pm")] + \
+               [("rho", _D), ("kqe", _D), ("comp", ctypes.c_char * STR), ("n_blobs", _I32), ("n_grid", _I32),
+                ("n_wav", _I32), ("n_theta", _I32), ("disks", Disk * MAXD), ("blobs", Blob * MAXB)] + \
+               [(n, _D) for n in ("amin", "amax", "mmin", "mmax", "f")] + \
+               [("seed", _I64), ("name_stub", ctypes.c_char * STR)]
+
+
+class RunInfo(ctypes.Structure):
+    _fields_ = [(n, _D) for n in ("time", "dt", "tlastd", "tlastf", "tlasts", "Rmax")] + \
+               [("n_mass", _I64), ("n_dust", _I64), ("n_all", _I64)]
+
+
+_lib = None
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int32)
+_PF = ctypes.POINTER(ctypes.c_float)
+
+
+class HostIOError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {lib().ddyn_host_error_text(code).decode()} (code {code})")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise RuntimeError(f"{HOST_LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(HOST_LIB_PATH)
+        L.ddyn_host_error_text.restype = ctypes.c_char_p
+        L.ddyn_host_error_text.argtypes = [ctypes.c_int]
+        L.ddyn_read_param.restype = ctypes.c_int
+        L.ddyn_read_param.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config)]
+        L.ddyn_det_step.restype = ctypes.c_double
+        L.ddyn_det_step.argtypes = [ctypes.POINTER(Config), ctypes.c_int64] + [_PD] * 4
+        L.ddyn_write_datafile.restype = ctypes.c_int
+        L.ddyn_write_datafile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Config),
+                                          ctypes.POINTER(RunInfo)] + [_PD] * 8 + [_PI, _PF, ctypes.c_int32, _PD]
+        L.ddyn_read_datafile.restype = ctypes.c_int
+        L.ddyn_read_datafile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Config),
+                                         ctypes.POINTER(RunInfo)] + [_PD] * 8 + [_PI, _PD]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_PD):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def read_param(path):
+    cfg = Config()
+    rc = lib().ddyn_read_param(str(path).encode(), ctypes.byref(cfg))
+    if rc:
+        raise HostIOError(rc, f"read_param({path})")
+    return cfg
+
+
+def det_step(cfg, st):
+    ng = st.n_grav
+    arr = [np.ascontiguousarray(a[:ng]) for a in (st.x, st.y, st.z, st.mu)]
+    return lib().ddyn_det_step(ctypes.byref(cfg), ng, *[_p(a) for a in arr])
+
+
+def state_from_config(cfg):
+    """Star + planets of a parsed .param as a DiskState."""
+    n = cfg.n_planets
+    st = make_state(n, 0)
+    for j in range(n):
+        st.x[j], st.y[j], st.z[j] = cfg.px[j], cfg.py[j], cfg.pz[j]
+        st.vx[j], st.vy[j], st.vz[j] = cfg.pvx[j], cfg.pvy[j], cfg.pvz[j]
+        st.mu[j] = cfg.pm[j]
+    return st
+
+
+def params_from_config(cfg, **kw):
+    """Physics parameters of the C ABI from a parsed configuration."""
+    return Params(n_planets=cfg.n_planets, SWQ=cfg.SWQ, FLQ=cfg.FLQ, vsw=cfg.vsw * YEAR / AU, Wz=cfg.Wz,
+                  Bstar=cfg.Bstar, Rstar=cfg.rstar, **kw)
+
+
+def write_datafile(path, cfg, st, run, beta_bins=None):
+    st.validate()
+    bb = None if beta_bins is None else np.ascontiguousarray(beta_bins, dtype=np.float32)
+    rc = lib().ddyn_write_datafile(str(path).encode(), VERSION.encode(), ctypes.byref(cfg), ctypes.byref(run),
+                                   *[_p(getattr(st, n)) for n in ("x", "y", "z", "vx", "vy", "vz", "N", "mu")],
+                                   _p(st.id, _PI), _p(bb, _PF), 0 if bb is None else len(bb), _p(st.beta_p))
+    if rc:
+        raise HostIOError(rc, f"write_datafile({path})")
+
+
+def read_datafile(path):
+    cfg, run = Config(), RunInfo()
+    rc = lib().ddyn_read_datafile(str(path).encode(), VERSION.encode(), ctypes.byref(cfg), ctypes.byref(run),
+                                  *([None] * 8), None, None)
+    if rc:
+        raise HostIOError(rc, f"read_datafile({path})")
+    n = run.n_all
+    arrs = {k: np.zeros(n) for k in ("x", "y", "z", "vx", "vy", "vz", "N", "mu", "beta")}
+    ids = np.zeros(n, dtype=np.int32)
+    rc = lib().ddyn_read_datafile(str(path).encode(), VERSION.encode(), ctypes.byref(cfg), ctypes.byref(run),
+                                  *[_p(arrs[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "N", "mu")],
+                                  _p(ids, _PI), _p(arrs["beta"]))
+    if rc:
+        raise HostIOError(rc, f"read_datafile({path})")
+    nd = run.n_dust
+    st = DiskState(n - nd, nd, arrs["x"], arrs["y"], arrs["z"], arrs["vx"], arrs["vy"], arrs["vz"], arrs["mu"],
+                   arrs["N"], ids, beta_p=np.ascontiguousarray(arrs["beta"][n - nd:]) if nd else None)
+    return cfg, run, st

@@ -1,0 +1,105 @@
+/*
+ * ddyn_host.h -- host-side envelope of the dynamical hot path (libddyn_host.so).
+ *
+ * Pure C++ (no GPU): the .param reader, the time-step rule, and the data-file
+ * writer/reader, with the record layout of the reference so files move both
+ * ways between the two codes.
+ *   ddyn_read_param      replaces read_in_param      (src/read_in.cu:508-733)
+ *                        with add_star, add_planets, add_disks, add_blobs and
+ *                        recalc_COM (src/read_in.cu:63-234, 344-375)
+ *   ddyn_det_step        replaces Grid::det_step     (src/grid.cu:312-342)
+ *   ddyn_write_datafile  replaces Grid::writegrid    (src/grid.cu:441-673)
+ *   ddyn_read_datafile   replaces read_in_model + Grid::read_in
+ *                        (src/read_in.cu:736-1014, src/grid.cu:205-310)
+ * Functions return 0 on success or the reference's error number (the index of
+ * its message table, src/error.cu:5-64); ddyn_host_error_text() gives a text.
+ * Units as in the reference's configdata after parsing.
+ */
+#ifndef DDYN_HOST_H
+#define DDYN_HOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDYN_MAX_PLANETS 64
+#define DDYN_MAX_DISKS 16
+#define DDYN_MAX_BLOBS 16
+#define DDYN_MAX_WAVS 64
+#define DDYN_STR 256
+
+typedef struct {
+    double Rin, Rout, dhdr, sigmae, amax, amin, eta, Mtot, mmax, mmin, gammas;
+    double dmax, dmmax, C;           /* optics-dependent; 0 until computed */
+    int64_t Ngrav, Nsmall;
+} ddyn_disk;
+
+typedef struct {
+    double a, ecc, inc, w, o, nu, sigma, amax, amin, eta, Mtot, mmax, mmin, vkepp;
+    double dmax, dmmax, C;
+    int64_t Ngrav, Nsmall;
+    int32_t collcloud, pad_;
+} ddyn_blob;
+
+typedef struct {
+    int32_t verb, gpuid, gl, n_step;
+    double t_gl, t_write, t_fits, t_sed, t_end;
+    char sptype[DDYN_STR];
+    double teff, rstar, Bstar, Wz, mloss, vsw;     /* rstar [AU], Bstar ref units, Wz [1/yr], vsw [m/s] */
+    double dist, fov, PA, inc, longitude;          /* dist [m], angles [rad] */
+    int32_t xsize, ysize, nwavs, radQ;
+    double wavs[DDYN_MAX_WAVS];                    /* [m] */
+    char specfile[DDYN_STR];
+    int32_t SWQ, FLQ, n_planets, n_disks;          /* n_planets counts the star */
+    /* star (index 0) and planets after recalc_COM; m = G*M */
+    double px[DDYN_MAX_PLANETS], py[DDYN_MAX_PLANETS], pz[DDYN_MAX_PLANETS];
+    double pvx[DDYN_MAX_PLANETS], pvy[DDYN_MAX_PLANETS], pvz[DDYN_MAX_PLANETS], pm[DDYN_MAX_PLANETS];
+    double rho, kqe;                               /* rho [kg/m^3], kqe per yr */
+    char comp[DDYN_STR];
+    int32_t n_blobs, n_grid, n_wav, n_theta;
+    ddyn_disk disks[DDYN_MAX_DISKS];
+    ddyn_blob blobs[DDYN_MAX_BLOBS];
+    double amin, amax, mmin, mmax, f;              /* dust grid (optics; 0 if unknown) */
+    int64_t seed;
+    char name_stub[DDYN_STR];
+} ddyn_config;
+
+/* Bookkeeping of a run, as Grid's time, dt, tlastd/f/s and counts. */
+typedef struct {
+    double time, dt, tlastd, tlastf, tlasts;
+    double Rmax;                      /* outer radius for the "## " marker */
+    int64_t n_mass, n_dust, n_all;    /* n_mass = swarm bodies (excludes star + planets) */
+} ddyn_run_info;
+
+const char *ddyn_host_error_text(int code);
+
+/* Parse a .param file (positional records 0-15). */
+int ddyn_read_param(const char *path, ddyn_config *cfg);
+
+/* Time step of Grid::det_step: min over massive bodies of 2 pi sqrt(r^3/(mu0+mui))/n_step
+ * and the disk/blob terms 2 pi sqrt(R/mu0)/n_step, as the reference writes them. */
+double ddyn_det_step(const ddyn_config *cfg, int64_t n_grav, const double *x, const double *y,
+                     const double *z, const double *mu);
+
+/* Write the data file.  Arrays are n_all long in the reference order; mu = G*M;
+ * beta_bin (n_bins floats, may be NULL) gives the beta column by size bin;
+ * beta_p (n_dust doubles, may be NULL) a per-grain beta column instead. */
+int ddyn_write_datafile(const char *path, const char *version, const ddyn_config *cfg,
+                        const ddyn_run_info *run, const double *x, const double *y, const double *z,
+                        const double *vx, const double *vy, const double *vz, const double *N,
+                        const double *mu, const int32_t *id, const float *beta_bin, int32_t n_bins,
+                        const double *beta_p);
+
+/* Read a data file.  Call once with x == NULL to fill cfg and run (counts);
+ * then again with arrays of run->n_all entries (beta may be NULL).  The
+ * configuration's n_planets is taken from the file.  Returns 55 on a version
+ * mismatch and 56 on an nR mismatch, as the reference does. */
+int ddyn_read_datafile(const char *path, const char *version, ddyn_config *cfg, ddyn_run_info *run,
+                       double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                       double *N, double *mu, int32_t *id, double *beta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
