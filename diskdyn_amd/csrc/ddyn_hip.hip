@@ -36,7 +36,8 @@ constexpr int kWG = 256;            // threads per workgroup (4 waves of 64)
 constexpr int kTile = 256;          // sources per LDS tile in the massive pair kernels
 constexpr int kDustTile32 = 1024;   // FP32 sources per LDS tile in the grain kernel (16 KiB)
 constexpr int kDustTile64 = 512;    // FP64 sources per LDS tile in the grain kernel (16 KiB)
-constexpr int kTargetWGs = 2048;    // pair-kernel workgroups wanted to fill 256 CUs
+constexpr int kTargetWGs = 16384;   // pair-kernel workgroups wanted (64 per CU: latency hiding)
+constexpr int kMinSlice = 2048;     // fewest sources per slice (bounds the partials' traffic)
 constexpr double kLightAU = 63239.6717367;   // src/ddyn.cuh:22
 constexpr double kFar = 1.0e15;     // position of padding sources/targets [AU]
 constexpr float kTiny2 = 1.0e-20f;  // FP32 r^2 floor for unsoftened pairs (keeps self pairs finite)
@@ -524,6 +525,178 @@ __global__ __launch_bounds__(kWG) void k_dust(DustArgs A)
 }
 
 // ---------------------------------------------------------------------------
+// Many steps in one launch for small massive sets (no swarm, n_grav <= 32)
+// ---------------------------------------------------------------------------
+// With only a star and a few planets the massive integration is a few hundred
+// FP64 operations per step, so every workgroup (one wave) integrates it
+// redundantly in LDS and advances its own 64 grains; nothing is exchanged
+// between workgroups and one launch covers many steps.  The arithmetic is that
+// of k_stage and k_dust (same expressions, same order), so results equal the
+// one-launch-per-kernel schedule bit for bit.
+constexpr int kSmallMax = 32;
+constexpr int kSmallWG = 64;
+constexpr int kSmallChunk = 1024;   // steps per launch of k_small_steps
+
+__global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps)
+{
+    __shared__ double4 P[4][kSmallMax];
+    const int tid = threadIdx.x;
+    const int ng = A.n_grav, np = A.n_planets;
+    const double dt = A.dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
+    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
+    // massive body tid (registers): base state, stage velocity and sums
+    double mx = 0, my = 0, mz = 0, mvx = 0, mvy = 0, mvz = 0, mmu = 0;
+    double vrx = 0, vry = 0, vrz = 0, Srx = 0, Sry = 0, Srz = 0, Sax = 0, Say = 0, Saz = 0;
+    if (tid < ng) {
+        mx = A.x[tid]; my = A.y[tid]; mz = A.z[tid];
+        mvx = A.vx[tid]; mvy = A.vy[tid]; mvz = A.vz[tid]; mmu = A.mu[tid];
+    }
+    // grain of this lane
+    const int g = blockIdx.x * kSmallWG + tid;
+    const int64_t t = (int64_t)ng + g;
+    bool live = false;
+    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0, m = 1, bet = 0, betsw = 0, qe = 0;
+    if (g < A.n_dust) {
+        live = A.N[t] != 0.0;
+        x0 = A.x[t]; y0 = A.y[t]; z0 = A.z[t];
+        vx0 = A.vx[t]; vy0 = A.vy[t]; vz0 = A.vz[t];
+        m = A.mu[t];
+        const int32_t id = A.id[t];
+        if (A.beta_p) {
+            bet = A.beta_p[g];
+            betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+        } else if (id >= 0 && id < A.n_bins) {
+            bet = (double)A.beta_t[id];
+            betsw = (double)A.betasw_t[id];
+        }
+        qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
+    }
+    bool removed = false;
+    for (int step = 0; step < nsteps; ++step) {
+        if (tid < ng) P[0][tid] = make_double4(mx, my, mz, mmu);
+        // massive stages 0-2 (k_stage<0..2>)
+        for (int s = 0; s < 3; ++s) {
+            __syncthreads();
+            if (tid < ng) {
+                const double4 pi = P[s][tid];
+                double ax = 0.0, ay = 0.0, az = 0.0;
+                for (int j = 0; j < np; ++j) {
+                    const double4 pj = P[s][j];
+                    const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                    const double f = r2 * __builtin_sqrt(r2);
+                    const double q = (f != 0.0) ? pj.w / f : 0.0;
+                    ax = __builtin_fma(dx, q, ax);
+                    ay = __builtin_fma(dy, q, ay);
+                    az = __builtin_fma(dz, q, az);
+                }
+                const double h = hs[s];
+                const double ux = s ? vrx : mvx, uy = s ? vry : mvy, uz = s ? vrz : mvz;
+                const double rx = __builtin_fma(ax, h, mvx), ry = __builtin_fma(ay, h, mvy),
+                             rz = __builtin_fma(az, h, mvz);
+                P[s + 1][tid] = make_double4(__builtin_fma(ux, h, mx), __builtin_fma(uy, h, my),
+                                             __builtin_fma(uz, h, mz), pi.w);
+                if (s == 0) {
+                    Srx = __builtin_fma(2.0, rx, mvx); Sry = __builtin_fma(2.0, ry, mvy);
+                    Srz = __builtin_fma(2.0, rz, mvz);
+                    Sax = ax; Say = ay; Saz = az;
+                } else if (s == 1) {
+                    Srx = __builtin_fma(2.0, rx, Srx); Sry = __builtin_fma(2.0, ry, Sry);
+                    Srz = __builtin_fma(2.0, rz, Srz);
+                    Sax = __builtin_fma(2.0, ax, Sax); Say = __builtin_fma(2.0, ay, Say);
+                    Saz = __builtin_fma(2.0, az, Saz);
+                } else {
+                    Srx = Srx + rx; Sry = Sry + ry; Srz = Srz + rz;
+                    Sax = __builtin_fma(2.0, ax, Sax); Say = __builtin_fma(2.0, ay, Say);
+                    Saz = __builtin_fma(2.0, az, Saz);
+                }
+                vrx = rx; vry = ry; vrz = rz;
+            }
+        }
+        __syncthreads();
+        // grains against the four stage positions (k_dust in the FP64 form)
+        if (live && !removed) {
+            double px = x0, py = y0, pz = z0, ux = vx0, uy = vy0, uz = vz0;
+            double kx[4], ky[4], kz[4], rx[4], ry[4], rz[4];
+            bool gone = false;
+            for (int s = 0; s < 4 && !gone; ++s) {
+                double ax = 0.0, ay = 0.0, az = 0.0;
+                gone = star_term(A, P[s][0], px, py, pz, ux, uy, uz, bet, betsw, qe, m, ax, ay, az);
+                bool hit = false;
+                grain_fp64(P[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
+                gone = gone || hit;
+                kx[s] = ax; ky[s] = ay; kz[s] = az;
+                if (s == 0) {
+                    rx[1] = __builtin_fma(ax, dt2, vx0); ry[1] = __builtin_fma(ay, dt2, vy0);
+                    rz[1] = __builtin_fma(az, dt2, vz0);
+                    px = __builtin_fma(vx0, dt2, x0); py = __builtin_fma(vy0, dt2, y0);
+                    pz = __builtin_fma(vz0, dt2, z0);
+                    ux = rx[1]; uy = ry[1]; uz = rz[1];
+                } else if (s == 1) {
+                    rx[2] = __builtin_fma(ax, dt2, vx0); ry[2] = __builtin_fma(ay, dt2, vy0);
+                    rz[2] = __builtin_fma(az, dt2, vz0);
+                    px = __builtin_fma(rx[1], dt2, x0); py = __builtin_fma(ry[1], dt2, y0);
+                    pz = __builtin_fma(rz[1], dt2, z0);
+                    ux = rx[2]; uy = ry[2]; uz = rz[2];
+                } else if (s == 2) {
+                    rx[3] = __builtin_fma(ax, dt, vx0); ry[3] = __builtin_fma(ay, dt, vy0);
+                    rz[3] = __builtin_fma(az, dt, vz0);
+                    px = __builtin_fma(rx[2], dt, x0); py = __builtin_fma(ry[2], dt, y0);
+                    pz = __builtin_fma(rz[2], dt, z0);
+                    ux = rx[3]; uy = ry[3]; uz = rz[3];
+                }
+            }
+            if (gone) {
+                removed = true;
+            } else {
+                const double nx0 = __builtin_fma(dt6, __builtin_fma(2.0, rx[1] + rx[2], vx0) + rx[3], x0);
+                const double ny0 = __builtin_fma(dt6, __builtin_fma(2.0, ry[1] + ry[2], vy0) + ry[3], y0);
+                const double nz0 = __builtin_fma(dt6, __builtin_fma(2.0, rz[1] + rz[2], vz0) + rz[3], z0);
+                vx0 = __builtin_fma(dt6, __builtin_fma(2.0, kx[1] + kx[2], kx[0]) + kx[3], vx0);
+                vy0 = __builtin_fma(dt6, __builtin_fma(2.0, ky[1] + ky[2], ky[0]) + ky[3], vy0);
+                vz0 = __builtin_fma(dt6, __builtin_fma(2.0, kz[1] + kz[2], kz[0]) + kz[3], vz0);
+                x0 = nx0; y0 = ny0; z0 = nz0;
+            }
+        }
+        __syncthreads();
+        // massive stage 3 = final combination (k_stage<3>)
+        if (tid < ng) {
+            const double4 pi = P[3][tid];
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int j = 0; j < np; ++j) {
+                const double4 pj = P[3][j];
+                const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                const double f = r2 * __builtin_sqrt(r2);
+                const double q = (f != 0.0) ? pj.w / f : 0.0;
+                ax = __builtin_fma(dx, q, ax);
+                ay = __builtin_fma(dy, q, ay);
+                az = __builtin_fma(dz, q, az);
+            }
+            mx = __builtin_fma(dt6, Srx, mx); my = __builtin_fma(dt6, Sry, my);
+            mz = __builtin_fma(dt6, Srz, mz);
+            mvx = __builtin_fma(dt6, Sax + ax, mvx);
+            mvy = __builtin_fma(dt6, Say + ay, mvy);
+            mvz = __builtin_fma(dt6, Saz + az, mvz);
+        }
+        __syncthreads();
+    }
+    if (g < A.n_dust && live) {
+        if (removed) {
+            A.N[t] = 0.0;
+        } else {
+            A.x[t] = x0; A.y[t] = y0; A.z[t] = z0;
+            A.vx[t] = vx0; A.vy[t] = vy0; A.vz[t] = vz0;
+        }
+    }
+    if (blockIdx.x == 0 && tid < ng) {
+        A.x[tid] = mx; A.y[tid] = my; A.z[tid] = mz;
+        A.vx[tid] = mvx; A.vy[tid] = mvy; A.vz[tid] = mvz;
+        store_pos(const_cast<double4*>(A.p64[0]), const_cast<float4*>(A.p32[0]), tid, mx, my, mz, mmu);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Discrete adjoint of the RK4 step
 // ---------------------------------------------------------------------------
 // Stage map of one step: R1 = r, V1 = v; R2 = r + h/2 V1, V2 = v + h/2 A1;
@@ -908,6 +1081,7 @@ struct ddyn_ctx {
     int32_t n_bins = 0;
     // timing
     bool timing = false;
+    bool fuse_small = true;      // k_small_steps for star + planets only systems
     struct Ev { int cls; hipEvent_t a, b; };
     std::vector<Ev> events;
     int64_t t_launch[DDYN_KCLASS_COUNT] = {0, 0, 0};
@@ -1005,7 +1179,7 @@ void plan_pairs(ddyn_ctx* c)
     if (ns <= 0) { c->nslice = 0; c->slice_len = 0; return; }
     const int tblocks = (int)((c->n_grav + (int64_t)kWG * c->tpt - 1) / ((int64_t)kWG * c->tpt));
     int want = (kTargetWGs + tblocks - 1) / tblocks;
-    const int max_slices = (ns + kTile - 1) / kTile;
+    const int max_slices = std::max(1, ns / kMinSlice);
     if (want > max_slices) want = max_slices;
     if (want < 1) want = 1;
     int len = (ns + want - 1) / want;
@@ -1185,6 +1359,7 @@ int ddyn_set_params(ddyn_ctx* c, const ddyn_params* p)
     if (c->x && p->n_planets > c->n_grav) return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
     c->prm = *p;
     c->have_params = true;
+    c->fuse_small = p->schedule == 0;
     if (c->x) {
         (void)hipSetDevice(c->device);
         const int old = c->nslice;
@@ -1284,6 +1459,18 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
     if (rc) return rc;
     if (!(dt > 0.0) || nsteps < 0) return fail(c, DDYN_ERR_ARG, "need dt > 0 and nsteps >= 0");
     DDYN_HIP(c, hipSetDevice(c->device));
+    if (n_swarm(c) == 0 && c->n_grav <= kSmallMax && c->fuse_small) {
+        // star + planets only: many steps per launch (bounded launch length)
+        const DustArgs A = dust_args(c, dt);
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + kSmallWG - 1) / kSmallWG);
+        for (int64_t k = 0; k < nsteps; k += kSmallChunk) {
+            const int n = (int)std::min<int64_t>(kSmallChunk, nsteps - k);
+            TimedLaunch tl(c, DDYN_KCLASS_DUST);
+            k_small_steps<<<blocks, kSmallWG, 0, c->stream>>>(A, n);
+            DDYN_HIP(c, hipGetLastError());
+        }
+        return DDYN_OK;
+    }
     for (int64_t k = 0; k < nsteps; ++k)
         if ((rc = one_step(c, dt))) return rc;
     return DDYN_OK;
@@ -1352,7 +1539,7 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
     const int nwaves = (int)((nd + kWG - 1) / kWG) * (kWG / 64);
     // VJP source split over [0, n_grav)
     const int tblocks = (int)((ng + kWG - 1) / kWG);
-    int vsl = std::max(1, std::min((kTargetWGs + tblocks - 1) / tblocks, (int)((ng + kTile - 1) / kTile)));
+    int vsl = std::max(1, std::min((kTargetWGs + tblocks - 1) / tblocks, (int)(ng / kMinSlice)));
     int vlen = (int)(((ng + vsl - 1) / vsl + kTile - 1) / kTile) * kTile;
     vsl = (int)((ng + vlen - 1) / vlen);
     DevBuf ckb, segb, finb, rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart;

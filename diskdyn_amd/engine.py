@@ -30,7 +30,7 @@ class DDynParams(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("eps", ctypes.c_double), ("Rmin", ctypes.c_double),
                 ("vsw", ctypes.c_double), ("Wz", ctypes.c_double), ("Bstar", ctypes.c_double),
                 ("Rstar", ctypes.c_double), ("light", ctypes.c_double),
-                ("fp64_radius", ctypes.c_double)]
+                ("fp64_radius", ctypes.c_double), ("schedule", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 SIGNATURES = {
@@ -98,7 +98,7 @@ def _fp(a):
 def to_c_params(p: Params) -> DDynParams:
     return DDynParams(int(p.n_planets), int(p.SWQ), int(p.FLQ), int(p.precision), float(p.eps),
                       float(p.Rmin), float(p.vsw), float(p.Wz), float(p.Bstar), float(p.Rstar),
-                      float(p.light), float(p.fp64_radius))
+                      float(p.light), float(p.fp64_radius), int(p.schedule), 0)
 
 
 class Engine:

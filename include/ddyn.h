@@ -61,6 +61,10 @@ typedef struct {
     double  light;       /* speed of light [AU/yr]; 0 selects lightAU = 63239.6717367 */
     double  fp64_radius; /* DDYN_FP32_FORCE: swarm pairs closer than this [AU] are evaluated in
                             FP64; <= 0 selects 10*eps */
+    int32_t schedule;    /* 0: fuse many steps per launch where the system allows it (star +
+                            planets only, <= 32 massive bodies); 1: one launch per kernel.
+                            Both give identical results. */
+    int32_t reserved;    /* must be 0 */
 } ddyn_params;
 
 /* Create a context on HIP device `device` (replaces checkcuda, src/cuda.cu:92-98). */

@@ -41,7 +41,7 @@ def test_binding_covers_header():
 
 def test_params_struct_layout():
     # int32 x4 then 8 doubles, no padding surprises
-    assert ctypes.sizeof(engine.DDynParams) == 16 + 8 * 8
+    assert ctypes.sizeof(engine.DDynParams) == 16 + 8 * 8 + 8
     assert engine.DDynParams.eps.offset == 16
     assert engine.DDynParams.fp64_radius.offset == 16 + 7 * 8
 

@@ -1,0 +1,93 @@
+"""End-to-end drop-in run of the ddyn driver on the GPU: .param in, data files
+out, the evolve loop's step clipping, and a -m restart from the written file.
+The final state is checked against the oracle stepped with the same dt sequence."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from diskdyn_amd import hostio
+from diskdyn_amd.system import Params
+from oracle import oracle as O
+from tests.test_hostio import PARAM
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DDYN = os.path.join(ROOT, "diskdyn_amd", "ddyn")
+
+# the test system without disks/blobs that would need particle generation
+PARAM_NO_DISKS = PARAM.replace("30.0 50.0 0.05 0.05 1.0 1.0 -3.5 0.1 -1.0 0 0", "30.0 50.0 0.05 0.05 1.0 1.0 -3.5 0.1 -1.0 0 0")
+
+
+def dt_sequence(t_write, t_fits, t_sed, t_end, dt, t0=0.0, tl=(0.0, 0.0, 0.0)):
+    """The step sizes of Grid::evolve's clipping loop (src/grid.cu:378-438)."""
+    time, (td, tf, ts) = t0, tl
+    seq = []
+
+    def clip(d):
+        if time + d > td + t_write:
+            d = t_write + td - time
+        if time + d > tf + t_fits:
+            d = t_fits + tf - time
+        if time + d > ts + t_sed:
+            d = t_sed + ts - time
+        if time + d > t_end:
+            d = t_end - time
+        return d
+
+    d = clip(dt)
+    while True:
+        seq.append(d)
+        time += d
+        reset = False
+        if time >= ts + t_sed or time >= t_end:
+            ts, reset = time, True
+        if time >= tf + t_fits or time >= t_end:
+            tf, reset = time, True
+        if time >= td + t_write or time >= t_end:
+            td, reset = time, True
+        if reset:
+            d = dt
+        d = clip(d)
+        if not time < t_end:
+            break
+    return seq
+
+
+def test_param_run_and_restart(tmp_path):
+    p = tmp_path / "sys.param"
+    p.write_text(PARAM_NO_DISKS.replace("20.0\n200\n", "2.0\n40\n").replace("5.0 10.0 10.0", "0.5 0.75 1.0"))
+    r = subprocess.run([DDYN, "-i", str(p), "--quiet"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    outs = sorted(f for f in os.listdir(tmp_path) if f.startswith("teststub_"))
+    assert "teststub_0.00.dat" in outs and "teststub_2.00.dat" in outs and "teststub_0.50.dat" in outs
+    cfg = hostio.read_param(p)
+    st = hostio.state_from_config(cfg)
+    dt = hostio.det_step(cfg, st)
+    seq = dt_sequence(cfg.t_write, cfg.t_fits, cfg.t_sed, cfg.t_end, dt)
+    prm = Params(n_planets=cfg.n_planets)
+    ref = st.copy()
+    for d in seq:
+        O.step(prm, ref, d, 1)
+    _, run, fin = hostio.read_datafile(tmp_path / "teststub_2.00.dat")
+    assert abs(run.time - 2.0) < 1e-12
+    err = np.abs(fin.pos() - ref.pos()).max() / np.abs(ref.pos()).max()
+    assert err < 1e-8          # file precision %.8e
+    # restart from the middle file and run to the end
+    (tmp_path / "restart").mkdir()
+    r = subprocess.run([DDYN, "-m", str(tmp_path / "teststub_1.00.dat"), "--quiet"], cwd=tmp_path / "restart",
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    _, run2, fin2 = hostio.read_datafile(tmp_path / "restart" / "teststub_2.00.dat")
+    assert np.abs(fin2.pos() - fin.pos()).max() < 1e-6 * np.abs(fin.pos()).max()
+
+
+def test_param_with_generated_particles_is_refused(tmp_path):
+    p = tmp_path / "d.param"
+    p.write_text(PARAM.replace("-3.5 0.1 -1.0 0 0", "-3.5 0.1 -1.0 0 100"))
+    r = subprocess.run([DDYN, "-i", str(p), "--quiet"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "not part of this build" in r.stderr

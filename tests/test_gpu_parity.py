@@ -89,6 +89,33 @@ def test_step_fp32(engine_cls, name, nsteps):
     assert np.median(ep) <= 1e-8
 
 
+@pytest.mark.parametrize("name,nsteps", [("cfg1", 300), ("cfg4", 50)])
+def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps):
+    """Star + planets only: the multi-step kernel equals one launch per kernel bit for bit."""
+    from dataclasses import replace
+    w = SMALL[name]()
+    st = w.state.copy()
+    # one grain inside Rmin, one exactly on a planet: removal must agree too
+    g0 = st.n_grav
+    st.x[g0], st.y[g0], st.z[g0] = st.x[0] + 0.01, st.y[0], st.z[0]
+    st.x[g0 + 1], st.y[g0 + 1], st.z[g0 + 1] = st.x[1], st.y[1], st.z[1]
+    prm = replace(w.params, Rmin=0.05)
+    out = []
+    for schedule in (0, 1):
+        s = st.copy()
+        with engine_cls() as eng:
+            eng.upload(s, replace(prm, schedule=schedule))
+            eng.step(w.dt, nsteps)
+            eng.download(s)
+        out.append(s)
+    print(f"[fused] {name}: removed {int((out[0].N == 0).sum())}")
+    for k in ("x", "y", "z", "vx", "vy", "vz", "N"):
+        assert np.array_equal(getattr(out[0], k), getattr(out[1], k)), k
+    ref = O.step(prm, st.copy(), w.dt, nsteps)
+    assert np.array_equal(out[0].N, ref.N)
+    assert pos_rel_err(out[0], ref).max() <= 1e-10
+
+
 def test_energy(engine_cls):
     w = SMALL["cfg2"]()
     with engine_cls() as eng:
