@@ -11,8 +11,6 @@ the massive G*M, beta (n_dust,) the per-grain radiation-pressure ratio.
 (src/read_in.cu:344-375) written in torch, so dL/d(m, a, e) follows by
 autograd from the adjoint's dL/d(initial state) and dL/dmu.
 """
-import math
-
 import numpy as np
 import torch
 
@@ -122,4 +120,23 @@ def ring_loss(rT, target, n_grav):
     return (d * d).sum(1).mean()
 
 
-__all__ = ["Runner", "Trajectory", "coe2rv_t", "planet_initial_state", "ring_loss", "math"]
+def kepler_targets(state, n_planets, t):
+    """Final positions of the swarm bodies on their unperturbed two-body orbits
+    around the star (mu = mu_star), as a fixed (detached) target."""
+    from .orbits import kepler_propagate_many
+    r0 = state.pos()[n_planets:] - state.pos()[0]
+    v0 = state.vel()[n_planets:] - state.vel()[0]
+    r, _ = kepler_propagate_many(state.mu[0], r0, v0, t)
+    out = state.pos().copy()
+    out[n_planets:] = r + state.pos()[0]
+    return torch.from_numpy(out)
+
+
+def kepler_deviation_loss(rT, target, n_planets):
+    """mean over swarm bodies of (|r_i(T)| - |r_i^Kep(T)|)^2."""
+    d = torch.linalg.norm(rT[n_planets:], dim=1) - torch.linalg.norm(target[n_planets:], dim=1)
+    return (d * d).mean()
+
+
+__all__ = ["Runner", "Trajectory", "coe2rv_t", "planet_initial_state", "ring_loss", "kepler_targets",
+           "kepler_deviation_loss"]

@@ -126,11 +126,41 @@ __device__ inline void tile_fp64(const double4* __restrict__ pos, int j0, int j1
     }
 }
 
+// FP32 sum of one LDS tile of kTile sources for TPT targets.  With SELF, the
+// tile holds the wave's own targets: the pair with source slot self0 + 64k
+// (the target itself) is masked to r^2 = 3e38, so it adds exactly zero
+// (inv^3 underflows) and is not seen by the close-pair minimum.
+template <int TPT, bool SELF>
+__device__ inline void tile_sum32(const float4* __restrict__ tile, int self0, const float* xf, const float* yf,
+                                  const float* zf, const float* e2f, float* fx, float* fy, float* fz, float* rmin)
+{
+#pragma unroll 4
+    for (int jj = 0; jj < kTile; ++jj) {
+        const float4 s = tile[jj];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            const float dx = s.x - xf[k], dy = s.y - yf[k], dz = s.z - zf[k];
+            float r2 = __builtin_fmaf(dx, dx, e2f[k]);
+            r2 = __builtin_fmaf(dy, dy, r2);
+            r2 = __builtin_fmaf(dz, dz, r2);
+            if (SELF) r2 = (jj == self0 + 64 * k) ? 3.0e38f : r2;
+            rmin[k] = fminf(rmin[k], r2);
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float inv2 = inv * inv;
+            const float q = (s.w * inv) * inv2;
+            fx[k] = __builtin_fmaf(dx, q, fx[k]);
+            fy[k] = __builtin_fmaf(dy, q, fy[k]);
+            fz[k] = __builtin_fmaf(dz, q, fz[k]);
+        }
+    }
+}
+
 // FP32-force form.  Each source tile is summed in FP32 (v_rsq_f32) into a
 // per-tile partial that is flushed into the FP64 accumulators.  A tile is
-// instead evaluated in FP64 for the whole wave when it holds one of the wave's
-// own targets (self pairs) or when any pair of the wave came closer than the
-// FP64 radius (r^2 < r64^2 + eps^2, tracked as a running minimum).
+// instead evaluated in FP64 for the whole wave when any pair of the wave came
+// closer than the FP64 radius (r^2 < r64^2 + eps^2, tracked as a running
+// minimum).  In a tile that holds the wave's own targets the self pairs are
+// masked out of both the sum and the minimum.
 template <int TPT>
 __global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos32,
                                                  const double4* __restrict__ pos64, int n_grav,
@@ -168,29 +198,17 @@ __global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos3
                                 ? pos32[j0 + threadIdx.x]
                                 : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
         __syncthreads();
-        bool redo = (j0 < wend) && (j0 + tl > wbase);   // wave-uniform: own targets in tile
-        if (!redo) {
+        // wave-uniform: does the tile hold one of this wave's own targets?
+        const bool own = (j0 < wend) && (j0 + tl > wbase);
+        bool redo = false;
+        {
             float fx[TPT], fy[TPT], fz[TPT], rmin[TPT];
 #pragma unroll
             for (int k = 0; k < TPT; ++k) { fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f; rmin[k] = 3.0e38f; }
-#pragma unroll 4
-            for (int jj = 0; jj < kTile; ++jj) {
-                const float4 s = tile[jj];
-#pragma unroll
-                for (int k = 0; k < TPT; ++k) {
-                    const float dx = s.x - xf[k], dy = s.y - yf[k], dz = s.z - zf[k];
-                    float r2 = __builtin_fmaf(dx, dx, e2f[k]);
-                    r2 = __builtin_fmaf(dy, dy, r2);
-                    r2 = __builtin_fmaf(dz, dz, r2);
-                    rmin[k] = fminf(rmin[k], r2);
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float inv2 = inv * inv;
-                    const float q = (s.w * inv) * inv2;
-                    fx[k] = __builtin_fmaf(dx, q, fx[k]);
-                    fy[k] = __builtin_fmaf(dy, q, fy[k]);
-                    fz[k] = __builtin_fmaf(dz, q, fz[k]);
-                }
-            }
+            if (own)
+                tile_sum32<TPT, true>(tile, wbase + lane - j0, xf, yf, zf, e2f, fx, fy, fz, rmin);
+            else
+                tile_sum32<TPT, false>(tile, 0, xf, yf, zf, e2f, fx, fy, fz, rmin);
             bool close = false;
 #pragma unroll
             for (int k = 0; k < TPT; ++k) close = close || (rmin[k] < thr[k]);
@@ -763,6 +781,125 @@ __global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ p
     }
 }
 
+// FP64 VJP of one source tile [j0, j1) for target k (self tiles and close pairs
+// of the FP32 form).  Sources come from global memory.
+__device__ inline void vjp_tile_fp64(const double4* __restrict__ pos, const double* __restrict__ Ab, size_t n,
+                                     int j0, int j1, int n_planets, double e2k, const double4 pk, double akx,
+                                     double aky, double akz, double& gx, double& gy, double& gz, double& gm)
+{
+    for (int j = j0; j < j1; ++j) {
+        const double4 pj = pos[j];
+        const double ajx = Ab[j], ajy = Ab[n + j], ajz = Ab[2 * n + j];
+        const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
+        const double s2 = dx * dx + dy * dy + dz * dz + ((j >= n_planets) ? e2k : 0.0);
+        if (s2 > 0.0) {
+            const double inv = 1.0 / __builtin_sqrt(s2);
+            const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            const double wx = pk.w * ajx - pj.w * akx, wy = pk.w * ajy - pj.w * aky, wz = pk.w * ajz - pj.w * akz;
+            const double dw = dx * wx + dy * wy + dz * wz;
+            gx += wx * inv3 - 3.0 * dx * dw * inv5;
+            gy += wy * inv3 - 3.0 * dy * dw * inv5;
+            gz += wz * inv3 - 3.0 * dz * dw * inv5;
+            gm -= (ajx * dx + ajy * dy + ajz * dz) * inv3;
+        }
+    }
+}
+
+// FP32 VJP sum over one LDS tile for one target; with SELF the source slot
+// self0 is the target itself and is masked (s^2 = 3e38: every term underflows
+// to zero and the close-pair minimum ignores it).
+template <bool SELF>
+__device__ inline void vjp_tile32(const float4* __restrict__ tp, const float4* __restrict__ ta, int self0,
+                                  float xf, float yf, float zf, float mk, float axf, float ayf, float azf,
+                                  float e2f, float& fx, float& fy, float& fz, float& fm, float& rmin)
+{
+#pragma unroll 2
+    for (int jj = 0; jj < kTile; ++jj) {
+        const float4 pj = tp[jj];
+        const float4 aj = ta[jj];
+        const float dx = pj.x - xf, dy = pj.y - yf, dz = pj.z - zf;
+        float s2 = __builtin_fmaf(dx, dx, e2f);
+        s2 = __builtin_fmaf(dy, dy, s2);
+        s2 = __builtin_fmaf(dz, dz, s2);
+        if (SELF) s2 = (jj == self0) ? 3.0e38f : s2;
+        rmin = fminf(rmin, s2);
+        const float inv = __builtin_amdgcn_rsqf(s2);
+        const float inv2 = inv * inv;
+        const float inv3 = inv2 * inv;
+        const float inv5 = inv3 * inv2;
+        const float wx = __builtin_fmaf(mk, aj.x, -pj.w * axf);
+        const float wy = __builtin_fmaf(mk, aj.y, -pj.w * ayf);
+        const float wz = __builtin_fmaf(mk, aj.z, -pj.w * azf);
+        const float dw = __builtin_fmaf(dz, wz, __builtin_fmaf(dy, wy, dx * wx));
+        const float t3 = 3.0f * dw * inv5;
+        fx = __builtin_fmaf(wx, inv3, __builtin_fmaf(-dx, t3, fx));
+        fy = __builtin_fmaf(wy, inv3, __builtin_fmaf(-dy, t3, fy));
+        fz = __builtin_fmaf(wz, inv3, __builtin_fmaf(-dz, t3, fz));
+        const float da = __builtin_fmaf(aj.z, dz, __builtin_fmaf(aj.y, dy, aj.x * dx));
+        fm = __builtin_fmaf(-da, inv3, fm);
+    }
+}
+
+// FP32-tile form of k_vjp_pairs: per 256-source tile an FP32 partial flushed
+// to FP64; the tile of star/planet sources and tiles where a pair of the wave
+// came closer than the FP64 radius are redone in FP64 (as k_pairs32).
+__global__ __launch_bounds__(kWG) void k_vjp_pairs32(const double4* __restrict__ pos,
+                                                     const double* __restrict__ Ab, int n_grav,
+                                                     int n_planets, int slice_len, double eps2,
+                                                     float r64sq, double* __restrict__ part)
+{
+    __shared__ float4 tp[kTile];
+    __shared__ float4 ta[kTile];
+    const size_t n = (size_t)n_grav;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wbase = blockIdx.x * kWG + wave * 64;
+    const int k = wbase + lane;
+    const bool ok = k < n_grav;
+    const double4 pk = ok ? pos[k] : make_double4(kFar, kFar, kFar, 0.0);
+    const double akx = ok ? Ab[k] : 0.0, aky = ok ? Ab[n + k] : 0.0, akz = ok ? Ab[2 * n + k] : 0.0;
+    const bool sw = k >= n_planets;
+    const double e2k = sw ? eps2 : 0.0;
+    const float xf = (float)pk.x, yf = (float)pk.y, zf = (float)pk.z, mk = (float)pk.w;
+    const float axf = (float)akx, ayf = (float)aky, azf = (float)akz;
+    const float e2f = sw ? fmaxf((float)eps2, kTiny2) : kTiny2;
+    const float thr = r64sq + e2f;
+    const int jbeg = blockIdx.y * slice_len;
+    const int jend = min(jbeg + slice_len, n_grav);
+    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
+    for (int j0 = jbeg; j0 < jend; j0 += kTile) {
+        const int tl = min(kTile, jend - j0);
+        __syncthreads();
+        if (threadIdx.x < tl) {
+            const int j = j0 + threadIdx.x;
+            const double4 pj = pos[j];
+            tp[threadIdx.x] = make_float4((float)pj.x, (float)pj.y, (float)pj.z, (float)pj.w);
+            ta[threadIdx.x] = make_float4((float)Ab[j], (float)Ab[n + j], (float)Ab[2 * n + j], 0.f);
+        } else {
+            tp[threadIdx.x] = make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
+            ta[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        // star/planet sources (unsoftened for swarm targets too): FP64
+        bool redo = j0 < n_planets;
+        if (!redo) {
+            float fx = 0.f, fy = 0.f, fz = 0.f, fm = 0.f, rmin = 3.0e38f;
+            const bool own = (j0 < wbase + 64) && (j0 + tl > wbase);   // wave-uniform
+            if (own)
+                vjp_tile32<true>(tp, ta, k - j0, xf, yf, zf, mk, axf, ayf, azf, e2f, fx, fy, fz, fm, rmin);
+            else
+                vjp_tile32<false>(tp, ta, 0, xf, yf, zf, mk, axf, ayf, azf, e2f, fx, fy, fz, fm, rmin);
+            redo = __any(ok && rmin < thr);
+            if (!redo) { gx += (double)fx; gy += (double)fy; gz += (double)fz; gm += (double)fm; }
+        }
+        if (redo && ok) vjp_tile_fp64(pos, Ab, n, j0, j0 + tl, n_planets, e2k, pk, akx, aky, akz, gx, gy, gz, gm);
+    }
+    if (ok) {
+        double* p = part + (size_t)blockIdx.y * 4 * n;
+        p[k] = gx; p[n + k] = gy; p[2 * n + k] = gz; p[3 * n + k] = gm;
+    }
+}
+
 struct AdjArgs {
     int n_grav, n_planets, n_dust, nslice, nwaves;
     size_t n_all;
@@ -1179,7 +1316,9 @@ void plan_pairs(ddyn_ctx* c)
     if (ns <= 0) { c->nslice = 0; c->slice_len = 0; return; }
     const int tblocks = (int)((c->n_grav + (int64_t)kWG * c->tpt - 1) / ((int64_t)kWG * c->tpt));
     int want = (kTargetWGs + tblocks - 1) / tblocks;
-    const int max_slices = std::max(1, ns / kMinSlice);
+    // few targets (a small swarm): slices down to one tile, the launch is latency-bound
+    const int min_slice = (tblocks < 64) ? kTile : kMinSlice;
+    const int max_slices = std::max(1, ns / min_slice);
     if (want > max_slices) want = max_slices;
     if (want < 1) want = 1;
     int len = (ns + want - 1) / want;
@@ -1604,8 +1743,16 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
             }
             const int wg = nd ? 1 : 0;
             for (int st = 3; st >= 0; --st) {
-                k_vjp_pairs<<<dim3(bg, (unsigned)vsl), kWG, 0, c->stream>>>(
-                    c->p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c), vpart.p);
+                if (c->prm.precision == DDYN_FP32_FORCE) {
+                    const double r64 = r64_of(c);
+                    k_vjp_pairs32<<<dim3(bg, (unsigned)vsl), kWG, 0, c->stream>>>(
+                        c->p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c),
+                        (float)(r64 * r64), vpart.p);
+                } else {
+                    k_vjp_pairs<<<dim3(bg, (unsigned)vsl), kWG, 0, c->stream>>>(
+                        c->p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c),
+                        vpart.p);
+                }
                 switch (st) {
                     case 3: k_adj_mstage<3><<<bg, kWG, 0, c->stream>>>(A, wg); break;
                     case 2: k_adj_mstage<2><<<bg, kWG, 0, c->stream>>>(A, wg); break;

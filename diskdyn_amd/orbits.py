@@ -91,6 +91,38 @@ def kepler_true_anomaly(M, e, iters=60):
     return 2.0 * np.arctan2(np.sqrt(1.0 + e) * np.sin(0.5 * E), np.sqrt(1.0 - e) * np.cos(0.5 * E))
 
 
+def kepler_propagate_many(mu, r0, v0, t, iters=50):
+    """Vectorised two-body propagation of bound orbits: r0, v0 (n, 3) around a
+    fixed centre of G*M = mu, advanced by time t; returns (r, v) (n, 3).
+    Elements -> eccentric anomaly (Newton on Kepler's equation) -> f and g."""
+    r0 = np.asarray(r0, dtype=np.float64)
+    v0 = np.asarray(v0, dtype=np.float64)
+    R = np.linalg.norm(r0, axis=1)
+    V2 = (v0 * v0).sum(1)
+    a = 1.0 / (2.0 / R - V2 / mu)
+    rv = (r0 * v0).sum(1)
+    n = np.sqrt(mu / a ** 3)
+    ecosE = 1.0 - R / a
+    esinE = rv / np.sqrt(mu * a)
+    e = np.hypot(ecosE, esinE)
+    E0 = np.arctan2(esinE, ecosE)
+    M1 = E0 - esinE + n * t
+    E = M1 + e * np.sin(M1)
+    for _ in range(iters):
+        dE = (E - e * np.sin(E) - M1) / (1.0 - e * np.cos(E))
+        E = E - dE
+        if np.all(np.abs(dE) < 1e-15):
+            break
+    dE = E - E0
+    f = 1.0 - a / R * (1.0 - np.cos(dE))
+    g = t - (dE - np.sin(dE)) / n
+    r1 = f[:, None] * r0 + g[:, None] * v0
+    R1 = np.linalg.norm(r1, axis=1)
+    fdot = -np.sqrt(mu * a) / (R * R1) * np.sin(dE)
+    gdot = 1.0 - a / R1 * (1.0 - np.cos(dE))
+    return r1, fdot[:, None] * r0 + gdot[:, None] * v0
+
+
 def kepler_propagate(mu, x, y, z, vx, vy, vz, t):
     """Exact two-body propagation of (r, v) around a fixed centre of G*M = mu by time t.
 

@@ -94,3 +94,19 @@ def test_interaction_counts():
     w = synth.config3(n_dust=262144, n_swarm=1024)
     assert w.interactions_per_eval() == 1026 * 1025 + 262144 * 1026
     assert synth.config1().interactions_per_eval() == 3 * 2 + 4096 * 3
+
+
+def test_kepler_propagate_many_matches_scalar_and_closes_orbits():
+    from diskdyn_amd import orbits
+    w = synth.config5(n_swarm=64)
+    st = w.state
+    r0, v0 = st.pos()[1:], st.vel()[1:]
+    r1, v1 = orbits.kepler_propagate_many(GRAV, r0, v0, 3.7)
+    for k in (0, 17, 63):
+        rs, vs = orbits.kepler_propagate(GRAV, *r0[k], *v0[k], 3.7)
+        assert np.allclose(r1[k], rs, rtol=1e-12, atol=1e-12) and np.allclose(v1[k], vs, rtol=1e-12, atol=1e-12)
+    # one full period returns to the start
+    a, _, _ = orbits.rv2coe(GRAV, 0.0, *r0[5], *v0[5])
+    P = 2 * math.pi * math.sqrt(a ** 3 / GRAV)
+    r2, _ = orbits.kepler_propagate_many(GRAV, r0[5:6], v0[5:6], P)
+    assert np.allclose(r2[0], r0[5], rtol=1e-10, atol=1e-10)
