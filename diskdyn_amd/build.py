@@ -54,9 +54,23 @@ def build_host(verbose=False):
     return HOST_LIB
 
 
+PROBE = os.path.join(HERE, "ddyn_probe")
+
+
+def build_probe(verbose=False):
+    """Rate microbenchmarks and the counter-calibration copy kernel (measurement)."""
+    tmp = PROBE + ".tmp"
+    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
+    _run([HIPCC] + flags + [os.path.join(CSRC, "ddyn_probe.hip"), "-o", tmp], "hipcc probe")
+    os.replace(tmp, PROBE)
+    if verbose:
+        print("built", PROBE)
+
+
 def build_all(verbose=False):
     build_hip(verbose)
     build_host(verbose)
+    build_probe(verbose)
 
 
 if __name__ == "__main__":
