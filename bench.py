@@ -36,7 +36,15 @@ PEAK_FP64_TFLOPS = 78.6
 
 
 def workload(name, precision):
-    if name == "cfg2":
+    if name == "cfg1":
+        w = synth.config1()
+        desc = ("small planetary system: 1 Msun star, planets at 5.2 and 9.5 AU, "
+                "4,096 massless test particles, dt=0.01 yr")
+    elif name == "cfg4":
+        w = synth.config4()
+        desc = ("adjoint ring forward: 1 Msun star, 1e-4 Msun planet at 25 AU, 32,768 grains "
+                "with radiation pressure and PR drag, dt=0.05 yr")
+    elif name == "cfg2":
         w = synth.config2(precision=precision)
         desc = "self-gravitating swarm: 1 Msun star + 65,536 bodies, eps=1e-3 AU, dt=0.02 yr"
     elif name == "cfg3":
@@ -86,6 +94,21 @@ def force_error(w, a_gpu, nsample=2048, seed=7):
     return float(np.median(err)), float(err.max())
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (each counter scaled by its
+    reading for a copy kernel of known size), or None if not profiled."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json")):
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("kernel") == kernel and d.get("workload") == workload:
+            scale_f = d["copy_kib_true"] / d["copy_fetch_kib_raw"]
+            scale_w = d["copy_kib_true"] / d["copy_write_kib_raw"]
+            return int(round((d["fetch_kib_raw"] * scale_f + d["write_kib_raw"] * scale_w) * 1024))
+    return None
+
+
 def cpu_baseline(w, target_s=12.0):
     """FP64 oracle on this host's cores: a bounded sample of force evaluations.
 
@@ -119,20 +142,113 @@ def cpu_baseline(w, target_s=12.0):
             "sample": sample + f" in {dt:.1f} s"}
 
 
+def adjoint_run(name):
+    """Discrete-adjoint workloads.
+
+    cfg4: 32,768-grain ring, 400 steps of dt = 0.05 yr, checkpoints every 20;
+          L = mean |r(T) - r_target(T)|^2 with the target from planet
+          m = 1.2e-4 Msun, a = 26 AU; gradients in planet (m, a, e) checked by
+          central finite differences (h = 1e-6 relative).
+    cfg5: 1,048,577-body disk in the FP32-force form, 10-step window,
+          checkpoints every 5; L = mean (|r(T)| - |r_Kepler(T)|)^2; directional
+          derivative in the initial state checked by central differences."""
+    import torch
+
+    from diskdyn_amd.adjoint import (Runner, Trajectory, kepler_deviation_loss, kepler_targets,
+                                     planet_initial_state, ring_loss)
+    out = {"metric": "discrete-adjoint RK4 steps/s (forward + reverse)", "unit": "steps/s", "n_gpus": 1,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic (seeded Philox)"}
+    if name == "cfg4":
+        nsteps, ck = 400, 20
+        base = synth.config4()
+        tgt = synth.config4(planet_m=1.2e-4, planet_a=26.0)
+        run = Runner(base.state, base.params, base.dt, nsteps, ckpt_every=ck)
+        target = torch.from_numpy(run.forward(tgt.state.copy()).pos())
+        ng = base.state.n_grav
+        beta = torch.from_numpy(base.state.beta_p.copy())
+
+        def loss(m, a, e):
+            r_m, v_m, mu = planet_initial_state(1.0, m, a, e)
+            r0 = torch.cat([r_m, torch.from_numpy(base.state.pos()[ng:])])
+            v0 = torch.cat([v_m, torch.from_numpy(base.state.vel()[ng:])])
+            rT, _ = Trajectory.apply(r0, v0, mu, beta, run)
+            return ring_loss(rT, target, ng)
+
+        x0 = {"m": 1e-4, "a": 25.0, "e": 0.15}
+        p = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in x0.items()}
+        t0 = time.perf_counter()
+        L = loss(p["m"], p["a"], p["e"])
+        L.backward()
+        t_adj = time.perf_counter() - t0
+        grads, fd_err = {}, {}
+        with torch.no_grad():
+            for k in x0:
+                h = 1e-6 * x0[k]
+                args_p = {q: torch.tensor(x0[q] + (h if q == k else 0.0), dtype=torch.float64) for q in x0}
+                args_m = {q: torch.tensor(x0[q] - (h if q == k else 0.0), dtype=torch.float64) for q in x0}
+                fd = (loss(**args_p).item() - loss(**args_m).item()) / (2 * h)
+                grads[k] = p[k].grad.item()
+                fd_err[k] = abs(grads[k] - fd) / abs(fd)
+        out.update({"value": nsteps / t_adj, "seconds": t_adj, "steps": nsteps, "ckpt_every": ck,
+                    "dtype": "fp64", "loss": L.item(), "grad": grads, "fd_rel_err": fd_err,
+                    "config": {"workload": "cfg4_adjoint: 1 Msun star, planet m=1e-4 Msun a=25 AU e=0.15, "
+                               "32,768 ring grains, beta~U(0,0.3), radiation + PR drag, dt=0.05 yr",
+                               "n_grav": ng, "n_dust": base.state.n_dust}})
+    else:
+        nsteps, ck = 10, 5
+        w = synth.config5(precision=FP32_FORCE)
+        st = w.state
+        run = Runner(st, w.params, w.dt, nsteps, ckpt_every=ck)
+        target = kepler_targets(st, w.params.n_planets, nsteps * w.dt)
+        empty = torch.zeros(0, dtype=torch.float64)
+
+        def loss(r0, v0, mu):
+            rT, _ = Trajectory.apply(r0, v0, mu, empty, run)
+            return kepler_deviation_loss(rT, target, w.params.n_planets)
+
+        r0 = torch.from_numpy(st.pos()).requires_grad_(True)
+        v0 = torch.from_numpy(st.vel()).requires_grad_(True)
+        mu = torch.from_numpy(st.mu.copy())
+        t0 = time.perf_counter()
+        L = loss(r0, v0, mu)
+        L.backward()
+        t_adj = time.perf_counter() - t0
+        rng = np.random.default_rng(3)
+        dr = torch.from_numpy(rng.normal(size=r0.shape))
+        dv = torch.from_numpy(rng.normal(size=v0.shape))
+        h = 1e-6
+        with torch.no_grad():
+            fd = (loss(r0 + h * dr, v0 + h * dv, mu).item() - loss(r0 - h * dr, v0 - h * dv, mu).item()) / (2 * h)
+        an = float((r0.grad * dr).sum() + (v0.grad * dv).sum())
+        out.update({"value": nsteps / t_adj, "seconds": t_adj, "steps": nsteps, "ckpt_every": ck,
+                    "dtype": "fp32 pair forces and VJP tiles, fp64 state", "loss": L.item(),
+                    "directional": {"adjoint": an, "fd": fd, "rel_err": abs(an - fd) / abs(fd)},
+                    "config": {"workload": "cfg5_large_disk 10-step adjoint window: 1 Msun star + 1,048,576 bodies, "
+                               "eps=5e-4 AU, dt=0.02 yr", "n_grav": st.n_grav, "n_dust": 0}})
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg2", "cfg3", "cfg5"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"])
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="steps for the plain FP64 comparison run (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
         return 2
+    if args.adjoint:
+        if args.workload not in ("cfg4", "cfg5"):
+            raise SystemExit("--adjoint runs the cfg4 or cfg5 workload")
+        return adjoint_run(args.workload)
 
     w32, desc = workload(args.workload, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
@@ -149,17 +265,22 @@ def main():
     # dominant kernel: massive pairs (swarm workloads) or the grain kernel (debris ring)
     dom = DDYN_KCLASS_PAIRS if w32.state.n_dust == 0 else DDYN_KCLASS_DUST
     nl, ms = kt32[dom]
+    ns = w32.state.n_grav - w32.params.n_planets
     if dom == DDYN_KCLASS_PAIRS:
-        ns = w32.state.n_grav - w32.params.n_planets
         inter_launch = w32.state.n_grav * ns
         kname = "k_pairs32"
+    elif ns == 0:
+        # star + planets only: k_small_steps runs all steps of a call in one launch
+        inter_launch = per_step * args.steps / max(nl, 1)
+        kname = "k_small_steps"
     else:
         inter_launch = 4 * w32.state.n_dust * w32.state.n_grav
         kname = "k_dust<fp32>"
     avg_ms = ms / max(nl, 1)
     achieved = FLOP_PER_INTERACTION * inter_launch / (avg_ms * 1e-3) / 1e12
     nl64, ms64 = kt64[dom]
-    ach64 = FLOP_PER_INTERACTION * inter_launch / (ms64 / max(nl64, 1) * 1e-3) / 1e12
+    inter64 = per_step * k64 / max(nl64, 1) if (ns == 0 and dom != DDYN_KCLASS_PAIRS) else inter_launch
+    ach64 = FLOP_PER_INTERACTION * inter64 / (ms64 / max(nl64, 1) * 1e-3) / 1e12
 
     out = {
         "metric": "pairwise interactions/s (FP32-force/FP64-state RK4)",
@@ -185,7 +306,7 @@ def main():
         "speedup_fp32_vs_fp64": value / v64,
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None, "launches": nl, "avg_launch_ms": avg_ms,
+                     "traffic": measured_traffic(kname, w32.name), "launches": nl, "avg_launch_ms": avg_ms,
                      "flop_per_launch": FLOP_PER_INTERACTION * inter_launch},
     }
     if not args.no_cpu:
