@@ -142,7 +142,7 @@ def cpu_baseline(w, target_s=12.0):
             "sample": sample + f" in {dt:.1f} s"}
 
 
-def adjoint_run(name):
+def adjoint_run(name, precision=FP32_FORCE):
     """Discrete-adjoint workloads.
 
     cfg4: 32,768-grain ring, 400 steps of dt = 0.05 yr, checkpoints every 20;
@@ -196,7 +196,7 @@ def adjoint_run(name):
                                "n_grav": ng, "n_dust": base.state.n_dust}})
     else:
         nsteps, ck = 10, 5
-        w = synth.config5(precision=FP32_FORCE)
+        w = synth.config5(precision=precision)
         st = w.state
         run = Runner(st, w.params, w.dt, nsteps, ckpt_every=ck)
         target = kepler_targets(st, w.params.n_planets, nsteps * w.dt)
@@ -221,7 +221,8 @@ def adjoint_run(name):
             fd = (loss(r0 + h * dr, v0 + h * dv, mu).item() - loss(r0 - h * dr, v0 - h * dv, mu).item()) / (2 * h)
         an = float((r0.grad * dr).sum() + (v0.grad * dv).sum())
         out.update({"value": nsteps / t_adj, "seconds": t_adj, "steps": nsteps, "ckpt_every": ck,
-                    "dtype": "fp32 pair forces and VJP tiles, fp64 state", "loss": L.item(),
+                    "dtype": ("fp32 pair forces and VJP tiles, fp64 state" if precision == FP32_FORCE
+                              else "fp64"), "loss": L.item(),
                     "directional": {"adjoint": an, "fd": fd, "rel_err": abs(an - fd) / abs(fd)},
                     "config": {"workload": "cfg5_large_disk 10-step adjoint window: 1 Msun star + 1,048,576 bodies, "
                                "eps=5e-4 AU, dt=0.02 yr", "n_grav": st.n_grav, "n_dust": 0}})
@@ -240,6 +241,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
+    ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -248,7 +250,7 @@ def main():
     if args.adjoint:
         if args.workload not in ("cfg4", "cfg5"):
             raise SystemExit("--adjoint runs the cfg4 or cfg5 workload")
-        return adjoint_run(args.workload)
+        return adjoint_run(args.workload, FP64 if args.fp64 else FP32_FORCE)
 
     w32, desc = workload(args.workload, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
