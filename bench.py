@@ -69,12 +69,15 @@ def timed_run(w, precision, steps, warmup):
         eng.synchronize()
         eng.set_timing(True)
         eng.synchronize()
+        e0 = eng.energy()[0]
+        eng.synchronize()
         t0 = time.perf_counter()
         eng.step(w.dt, steps)
         eng.synchronize()
         t1 = time.perf_counter()
         kt = {k: eng.read_timing(k) for k in (DDYN_KCLASS_PAIRS, DDYN_KCLASS_DUST)}
-    return t1 - t0, kt, a
+        e1 = eng.energy()[0]
+    return t1 - t0, kt, a, abs((e1 - e0) / e0)
 
 
 def force_error(w, a_gpu, nsample=2048, seed=7):
@@ -149,9 +152,11 @@ def adjoint_run(name, precision=FP32_FORCE):
           L = mean |r(T) - r_target(T)|^2 with the target from planet
           m = 1.2e-4 Msun, a = 26 AU; gradients in planet (m, a, e) checked by
           central finite differences (h = 1e-6 relative).
-    cfg5: 1,048,577-body disk in the FP32-force form, 10-step window,
-          checkpoints every 5; L = mean (|r(T)| - |r_Kepler(T)|)^2; directional
-          derivative in the initial state checked by central differences."""
+    cfg5: 1,048,577-body disk, 10-step window; FP32-force form with
+          checkpoints every 5, or the plain FP64 form (--fp64) with checkpoints
+          every 2; L = mean (|r(T)| - |r_Kepler(T)|)^2; directional derivative
+          in the initial state checked by central differences.  Progress of
+          each forward chunk and adjoint window goes to stderr."""
     import torch
 
     from diskdyn_amd.adjoint import (Runner, Trajectory, kepler_deviation_loss, kepler_targets,
@@ -195,10 +200,12 @@ def adjoint_run(name, precision=FP32_FORCE):
                                "32,768 ring grains, beta~U(0,0.3), radiation + PR drag, dt=0.05 yr",
                                "n_grav": ng, "n_dust": base.state.n_dust}})
     else:
-        nsteps, ck = 10, 5
+        nsteps = 10
+        ck = 5 if precision == FP32_FORCE else 2
         w = synth.config5(precision=precision)
         st = w.state
-        run = Runner(st, w.params, w.dt, nsteps, ckpt_every=ck)
+        report = lambda msg: print(f"[adjoint cfg5] {msg}", file=sys.stderr, flush=True)  # noqa: E731
+        run = Runner(st, w.params, w.dt, nsteps, ckpt_every=ck, report=report)
         target = kepler_targets(st, w.params.n_planets, nsteps * w.dt)
         empty = torch.zeros(0, dtype=torch.float64)
 
@@ -255,9 +262,9 @@ def main():
     w32, desc = workload(args.workload, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
 
-    t32, kt32, a32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup)
+    t32, kt32, a32, drift32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup)
     k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
-    t64, kt64, a64 = timed_run(w32, FP64, k64, min(args.warmup, 2))
+    t64, kt64, a64, drift64 = timed_run(w32, FP64, k64, min(args.warmup, 2))
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)
@@ -306,6 +313,8 @@ def main():
                  "force_rel_err_median": med64, "force_rel_err_max": max64,
                  "kernel_tflops": ach64, "kernel_frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS},
         "speedup_fp32_vs_fp64": value / v64,
+        "energy_drift": {"fp32_force": drift32, "fp64": drift64,
+                         "note": "|dE/E0| of the massive set over the timed steps (each form's own step count)"},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(kname, w32.name), "launches": nl, "avg_launch_ms": avg_ms,

@@ -22,9 +22,10 @@ class Runner:
     """Fixed parts of a run: the template state (ids, N, beta_sw, grain masses),
     parameters, step size, step count and checkpoint interval."""
 
-    def __init__(self, state, params, dt, nsteps, ckpt_every=0, device=0):
+    def __init__(self, state, params, dt, nsteps, ckpt_every=0, device=0, report=None):
         self.state, self.params = state, params
         self.dt, self.nsteps, self.ckpt_every, self.device = dt, nsteps, ckpt_every, device
+        self.report = report
 
     def make_state(self, r0, v0, mu, beta):
         st = self.state.copy()
@@ -43,14 +44,45 @@ class Runner:
     def forward(self, st):
         with Engine(self.device) as eng:
             eng.upload(st, self.params)
-            eng.step(self.dt, self.nsteps)
+            if self.report is None or not self.ckpt_every:
+                eng.step(self.dt, self.nsteps)
+            else:
+                for k0 in range(0, self.nsteps, self.ckpt_every):
+                    eng.step(self.dt, min(self.ckpt_every, self.nsteps - k0))
+                    eng.synchronize()
+                    self.report(f"forward step {min(k0 + self.ckpt_every, self.nsteps)}/{self.nsteps}")
             out = eng.download(st.copy())
         return out
 
     def adjoint(self, st, lam_r, lam_v):
+        """dL/d(initial r, v), dL/dmu and dL/dbeta for dL/d(final r, v) = lam_r, lam_v.
+
+        With `report` set, the reverse sweep is chained segment by segment
+        (ddyn_adjoint on each checkpoint window, the adjoint state passed from
+        one window to the previous) and `report` is called after each window."""
+        if self.report is None or not self.ckpt_every or self.ckpt_every >= self.nsteps:
+            with Engine(self.device) as eng:
+                eng.upload(st, self.params)
+                return eng.adjoint(self.dt, self.nsteps, lam_r, lam_v, self.ckpt_every)
+        ck = self.ckpt_every
+        starts = list(range(0, self.nsteps, ck))
         with Engine(self.device) as eng:
             eng.upload(st, self.params)
-            return eng.adjoint(self.dt, self.nsteps, lam_r, lam_v, self.ckpt_every)
+            ckpts = []
+            for k0 in starts:                      # forward pass, host checkpoints
+                ckpts.append(eng.download(st.copy()))
+                eng.step(self.dt, min(ck, self.nsteps - k0))
+            self.report(f"forward {self.nsteps} steps done")
+            g_mu = np.zeros(st.n_grav)
+            g_beta = np.zeros(st.n_dust)
+            for i in range(len(starts) - 1, -1, -1):
+                n = min(ck, self.nsteps - starts[i])
+                eng.upload(ckpts[i], self.params)
+                lam_r, lam_v, gm, gb = eng.adjoint(self.dt, n, lam_r, lam_v, n)
+                g_mu += gm
+                g_beta += gb
+                self.report(f"adjoint window {len(starts) - i}/{len(starts)} done")
+        return lam_r, lam_v, g_mu, g_beta
 
 
 class Trajectory(torch.autograd.Function):

@@ -67,6 +67,21 @@ def test_gpu_adjoint_fp32_tiles(engine_cls):
     assert max(errs.values()) <= 1e-5
 
 
+def test_segment_chained_adjoint_equals_single_call():
+    """Runner with a progress report chains ddyn_adjoint window by window; the
+    gradients equal one ddyn_adjoint call over all steps."""
+    w = synth.config3(n_dust=300, n_swarm=16, precision=FP64, eps=2e-2)
+    st = w.state
+    rng = np.random.default_rng(21)
+    lam_r, lam_v = rng.normal(size=(st.n_all, 3)), rng.normal(size=(st.n_all, 3))
+    msgs = []
+    a = Runner(st, w.params, w.dt, 9, ckpt_every=4).adjoint(st.copy(), lam_r, lam_v)
+    b = Runner(st, w.params, w.dt, 9, ckpt_every=4, report=msgs.append).adjoint(st.copy(), lam_r, lam_v)
+    assert len(msgs) == 4   # forward + 3 windows
+    for x, y in zip(a, b):
+        assert _rel(np.asarray(y), np.asarray(x)) <= 1e-12
+
+
 def test_gpu_adjoint_directional_fd(engine_cls):
     w = synth.config4(n_dust=128)
     st = w.state
