@@ -34,6 +34,10 @@ namespace {
 
 constexpr int kWG = 256;            // threads per workgroup (4 waves of 64)
 constexpr int kTile = 256;          // sources per LDS tile in the massive pair kernels
+#ifndef DDYN_DUST_WG
+#define DDYN_DUST_WG 256
+#endif
+constexpr int kDustWG = DDYN_DUST_WG;   // threads per workgroup of the grain kernel (one grain each)
 constexpr int kDustTile32 = 1024;   // FP32 sources per LDS tile in the grain kernel (16 KiB)
 constexpr int kDustTile64 = 512;    // FP64 sources per LDS tile in the grain kernel (16 KiB)
 constexpr int kTargetWGs = 16384;   // pair-kernel workgroups wanted (64 per CU: latency hiding)
@@ -402,11 +406,11 @@ __device__ inline void grain_fp64(const double4* __restrict__ P, int j0, int j1,
 }
 
 template <bool F32, bool ACCEL>
-__global__ __launch_bounds__(kWG) void k_dust(DustArgs A)
+__global__ __launch_bounds__(kDustWG) void k_dust(DustArgs A)
 {
     __shared__ double4 lds64[kDustTile64];   // 16 KiB; the FP32 form views it as float4[1024]
     float4* lds32 = reinterpret_cast<float4*>(lds64);
-    const int g = blockIdx.x * kWG + threadIdx.x;
+    const int g = blockIdx.x * kDustWG + threadIdx.x;
     const int64_t t = (int64_t)A.n_grav + g;
     bool live = false;
     double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0, m = 1, bet = 0, betsw = 0, qe = 0;
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(kWG) void k_dust(DustArgs A)
             for (int j0 = A.n_planets; j0 < A.n_grav; j0 += kDustTile32) {
                 const int tl = min(kDustTile32, A.n_grav - j0);
                 __syncthreads();
-                for (int q = threadIdx.x; q < kDustTile32; q += kWG)
+                for (int q = threadIdx.x; q < kDustTile32; q += kDustWG)
                     lds32[q] = (q < tl) ? A.p32[s][j0 + q]
                                         : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
                 __syncthreads();
@@ -482,7 +486,7 @@ __global__ __launch_bounds__(kWG) void k_dust(DustArgs A)
             for (int j0 = A.n_planets; j0 < A.n_grav; j0 += kDustTile64) {
                 const int tl = min(kDustTile64, A.n_grav - j0);
                 __syncthreads();
-                for (int q = threadIdx.x; q < kDustTile64; q += kWG)
+                for (int q = threadIdx.x; q < kDustTile64; q += kDustWG)
                     lds64[q] = (q < tl) ? P[j0 + q] : make_double4(kFar, kFar, kFar, 0.0);
                 __syncthreads();
                 if (live && !gone) {
@@ -1421,14 +1425,14 @@ int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel)
 {
     if (c->n_dust == 0) return DDYN_OK;
     TimedLaunch tl(c, DDYN_KCLASS_DUST);
-    const unsigned blocks = (unsigned)((c->n_dust + kWG - 1) / kWG);
+    const unsigned blocks = (unsigned)((c->n_dust + kDustWG - 1) / kDustWG);
     const bool f32 = c->prm.precision == DDYN_FP32_FORCE;
     if (accel) {
-        if (f32) k_dust<true, true><<<blocks, kWG, 0, c->stream>>>(A);
-        else k_dust<false, true><<<blocks, kWG, 0, c->stream>>>(A);
+        if (f32) k_dust<true, true><<<blocks, kDustWG, 0, c->stream>>>(A);
+        else k_dust<false, true><<<blocks, kDustWG, 0, c->stream>>>(A);
     } else {
-        if (f32) k_dust<true, false><<<blocks, kWG, 0, c->stream>>>(A);
-        else k_dust<false, false><<<blocks, kWG, 0, c->stream>>>(A);
+        if (f32) k_dust<true, false><<<blocks, kDustWG, 0, c->stream>>>(A);
+        else k_dust<false, false><<<blocks, kDustWG, 0, c->stream>>>(A);
     }
     DDYN_HIP(c, hipGetLastError());
     return DDYN_OK;
