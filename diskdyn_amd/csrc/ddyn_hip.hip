@@ -34,12 +34,9 @@ namespace {
 
 constexpr int kWG = 256;            // threads per workgroup (4 waves of 64)
 constexpr int kTile = 256;          // sources per LDS tile in the massive pair kernels
-#ifndef DDYN_DUST_WG
-#define DDYN_DUST_WG 256
-#endif
-constexpr int kDustWG = DDYN_DUST_WG;   // threads per workgroup of the grain kernel (one grain each)
-constexpr int kDustTile32 = 1024;   // FP32 sources per LDS tile in the grain kernel (16 KiB)
-constexpr int kDustTile64 = 512;    // FP64 sources per LDS tile in the grain kernel (16 KiB)
+constexpr int kDustSplit = 4;       // grain kernel: waves per workgroup, each over 1/4 of the swarm sources
+constexpr int kDustWG = 64 * kDustSplit;   // 64 grains per workgroup, one per lane
+constexpr int kDustTile = 256;      // grain kernel: sources per FP32 partial and close-pair test
 constexpr int kTargetWGs = 16384;   // pair-kernel workgroups wanted (64 per CU: latency hiding)
 constexpr int kMinSlice = 2048;     // fewest sources per slice (bounds the partials' traffic)
 constexpr double kLightAU = 63239.6717367;   // src/ddyn.cuh:22
@@ -405,145 +402,196 @@ __device__ inline void grain_fp64(const double4* __restrict__ P, int j0, int j1,
     }
 }
 
+// Sources are read through scalar loads: every lane of a wave sweeps the same
+// source index, so a constant-address-space view of the stage positions lets
+// the compiler fetch them with s_load and feed the VALU from SGPRs (no LDS
+// staging and no barriers in the source loop).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const f4v cf4v;
+typedef __attribute__((address_space(4))) const d4v cd4v;
+
+// Per-workgroup grain records.  Wave 0 owns the RK4 bookkeeping; the other
+// waves read the current stage position from here and post their partial
+// accelerations.  Keeping this state in LDS instead of registers keeps the
+// source loop at <= 64 VGPRs (8 waves per SIMD).
+struct DustShared {
+    double x0[3][64], v0[3][64];     // state at the start of the step
+    double p[3][64], u[3][64];       // current stage position / velocity
+    double k0[3][64], k12[3][64];    // k1 and k2 + k3 (accelerations)
+    double r12[3][64];               // r2 + r3 (stage velocities)
+    double a[kDustSplit][3][64];     // per-wave partial accelerations
+    int32_t flag[kDustSplit][64];    // per-wave "grain removed" flags
+};
+
 template <bool F32, bool ACCEL>
-__global__ __launch_bounds__(kDustWG) void k_dust(DustArgs A)
+#ifndef DDYN_DUST_WPE
+#define DDYN_DUST_WPE 7
+#endif
+__global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(DDYN_DUST_WPE, 8))) void k_dust(DustArgs A)
 {
-    __shared__ double4 lds64[kDustTile64];   // 16 KiB; the FP32 form views it as float4[1024]
-    float4* lds32 = reinterpret_cast<float4*>(lds64);
-    const int g = blockIdx.x * kDustWG + threadIdx.x;
+    __shared__ DustShared S;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = blockIdx.x * 64 + lane;
     const int64_t t = (int64_t)A.n_grav + g;
-    bool live = false;
-    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0, m = 1, bet = 0, betsw = 0, qe = 0;
-    if (g < A.n_dust) {
-        live = ACCEL || A.N[t] != 0.0;
-        x0 = A.x[t]; y0 = A.y[t]; z0 = A.z[t];
-        vx0 = A.vx[t]; vy0 = A.vy[t]; vz0 = A.vz[t];
-        m = A.mu[t];
-        const int32_t id = A.id[t];
-        if (A.beta_p) {
-            bet = A.beta_p[g];
-            betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
-        } else if (id >= 0 && id < A.n_bins) {
-            bet = (double)A.beta_t[id];
-            betsw = (double)A.betasw_t[id];
+    const bool valid = g < A.n_dust;
+    const bool live = valid && (ACCEL || A.N[t] != 0.0);
+    // this wave's share of the swarm sources, in whole tiles
+    const int ns = A.n_grav - A.n_planets;
+    const int per = ((ns + kDustSplit * kDustTile - 1) / (kDustSplit * kDustTile)) * kDustTile;
+    const int jb = A.n_planets + min(ns, wave * per);
+    const int je = A.n_planets + min(ns, (wave + 1) * per);
+    const int nparts = per > 0 ? (ns + per - 1) / per : 1;   // waves holding sources (wave 0 always)
+    const double dt = A.dt, dt2 = dt / 2.0;
+    if (wave == 0) {
+        const double xs[3] = {valid ? A.x[t] : kFar, valid ? A.y[t] : kFar, valid ? A.z[t] : kFar};
+        const double vs[3] = {valid ? A.vx[t] : 0.0, valid ? A.vy[t] : 0.0, valid ? A.vz[t] : 0.0};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            S.x0[c][lane] = xs[c]; S.v0[c][lane] = vs[c];
+            S.p[c][lane] = xs[c]; S.u[c][lane] = vs[c];
         }
-        qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
     }
-    const double dt = A.dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
-    double px = x0, py = y0, pz = z0, ux = vx0, uy = vy0, uz = vz0;   // stage position, velocity
-    double kx[4], ky[4], kz[4], rx[4], ry[4], rz[4];
+    __syncthreads();
     bool gone = false;
     constexpr int kStages = ACCEL ? 1 : 4;
 #pragma unroll 1
     for (int s = 0; s < kStages; ++s) {
-        const double4* P = A.p64[s];
+        const double px = S.p[0][lane], py = S.p[1][lane], pz = S.p[2][lane];
+        const bool act = live && !gone;
         double ax = 0.0, ay = 0.0, az = 0.0;
-        if (live && !gone) {
-            gone = star_term(A, P[0], px, py, pz, ux, uy, uz, bet, betsw, qe, m, ax, ay, az);
-            bool hit = false;
-            grain_fp64(P, 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
-            gone = gone || hit;
+        bool hit = false;
+        if (wave == 0 && act) {
+            // star and planets (src/gpu_functions.cu:679-730), FP64
+            double m = A.mu[t], bet = 0.0, betsw = 0.0, qe = 0.0;
+            const int32_t id = A.id[t];
+            if (A.beta_p) {
+                bet = A.beta_p[g];
+                betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+            } else if (id >= 0 && id < A.n_bins) {
+                bet = (double)A.beta_t[id];
+                betsw = (double)A.betasw_t[id];
+            }
+            qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
+            hit = star_term(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane], bet, betsw,
+                            qe, m, ax, ay, az);
+            grain_fp64(A.p64[s], 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
         }
-        // Swarm sources, LDS tiles (all threads take part in the barriers).
-        if (F32) {
-            const float xf = (float)px, yf = (float)py, zf = (float)pz;
-            const float e2f = fmaxf(A.eps2f, kTiny2);
-            for (int j0 = A.n_planets; j0 < A.n_grav; j0 += kDustTile32) {
-                const int tl = min(kDustTile32, A.n_grav - j0);
-                __syncthreads();
-                for (int q = threadIdx.x; q < kDustTile32; q += kDustWG)
-                    lds32[q] = (q < tl) ? A.p32[s][j0 + q]
-                                        : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
-                __syncthreads();
-                const bool act = live && !gone;
-                float fx = 0.f, fy = 0.f, fz = 0.f, rmin = 3.0e38f;
-#pragma unroll 4
-                for (int jj = 0; jj < kDustTile32; ++jj) {
-                    const float4 sj = lds32[jj];
-                    const float dx = sj.x - xf, dy = sj.y - yf, dz = sj.z - zf;
-                    float r2 = __builtin_fmaf(dx, dx, e2f);
-                    r2 = __builtin_fmaf(dy, dy, r2);
-                    r2 = __builtin_fmaf(dz, dz, r2);
-                    rmin = fminf(rmin, r2);
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float inv2 = inv * inv;
-                    const float q = (sj.w * inv) * inv2;
-                    fx = __builtin_fmaf(dx, q, fx);
-                    fy = __builtin_fmaf(dy, q, fy);
-                    fz = __builtin_fmaf(dz, q, fz);
-                }
-                const bool redo = __any(act && rmin < A.thr2f);
-                if (act) {
-                    if (!redo) {
-                        ax += (double)fx; ay += (double)fy; az += (double)fz;
+        if (__builtin_amdgcn_readfirstlane(__any(act)) && jb < je) {
+            if (F32) {
+                const cf4v* src = (const cf4v*)A.p32[s];
+                const float xf = (float)px, yf = (float)py, zf = (float)pz;
+                const float e2f = fmaxf(A.eps2f, kTiny2);
+                for (int j0 = jb; j0 < je; j0 += kDustTile) {
+                    const int tl = min(kDustTile, je - j0);
+                    float fx = 0.f, fy = 0.f, fz = 0.f, rmin = 3.0e38f;
+                    auto pull = [&](const f4v sj) {
+                        const float dx = sj.x - xf, dy = sj.y - yf, dz = sj.z - zf;
+                        float r2 = __builtin_fmaf(dx, dx, e2f);
+                        r2 = __builtin_fmaf(dy, dy, r2);
+                        r2 = __builtin_fmaf(dz, dz, r2);
+                        rmin = fminf(rmin, r2);
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float inv2 = inv * inv;
+                        const float q = (sj.w * inv) * inv2;
+                        fx = __builtin_fmaf(dx, q, fx);
+                        fy = __builtin_fmaf(dy, q, fy);
+                        fz = __builtin_fmaf(dz, q, fz);
+                    };
+                    if (tl == kDustTile) {
+#pragma unroll 8
+                        for (int jj = 0; jj < kDustTile; ++jj) pull(src[j0 + jj]);
                     } else {
-                        bool hit = false;
-                        grain_fp64(P, j0, j0 + tl, px, py, pz, A.eps2, ax, ay, az, hit);
+                        for (int jj = 0; jj < tl; ++jj) pull(src[j0 + jj]);
+                    }
+                    const bool redo = __any(act && rmin < A.thr2f);
+                    if (act) {
+                        if (!redo) {
+                            ax += (double)fx; ay += (double)fy; az += (double)fz;
+                        } else {
+                            grain_fp64(A.p64[s], j0, j0 + tl, px, py, pz, A.eps2, ax, ay, az, hit);
+                        }
                     }
                 }
-            }
-        } else {
-            for (int j0 = A.n_planets; j0 < A.n_grav; j0 += kDustTile64) {
-                const int tl = min(kDustTile64, A.n_grav - j0);
-                __syncthreads();
-                for (int q = threadIdx.x; q < kDustTile64; q += kDustWG)
-                    lds64[q] = (q < tl) ? P[j0 + q] : make_double4(kFar, kFar, kFar, 0.0);
-                __syncthreads();
-                if (live && !gone) {
-                    bool hit = false;
-                    for (int jj = 0; jj < tl; ++jj) {
-                        const double4 sj = lds64[jj];
-                        const double ex = px - sj.x, ey = py - sj.y, ez = pz - sj.z;
-                        const double RRu = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
-                        hit = hit || (RRu == 0.0);
-                        const double R2 = RRu + A.eps2;
-                        const double f = sj.w / (R2 * __builtin_sqrt(R2));
-                        ax = __builtin_fma(-f, ex, ax);
-                        ay = __builtin_fma(-f, ey, ay);
-                        az = __builtin_fma(-f, ez, az);
-                    }
-                    gone = gone || hit;
+            } else {
+                const cd4v* src = (const cd4v*)A.p64[s];
+                const double e2 = A.eps2;
+#pragma unroll 4
+                for (int j = jb; j < je; ++j) {
+                    const d4v sj = src[j];
+                    const double ex = px - sj.x, ey = py - sj.y, ez = pz - sj.z;
+                    const double RRu = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
+                    hit = hit || (RRu == 0.0);
+                    const double R2 = RRu + e2;
+                    const double f = sj.w / (R2 * __builtin_sqrt(R2));
+                    ax = __builtin_fma(-f, ex, ax);
+                    ay = __builtin_fma(-f, ey, ay);
+                    az = __builtin_fma(-f, ez, az);
                 }
             }
         }
-        kx[s] = ax; ky[s] = ay; kz[s] = az;
-        if (ACCEL) break;
-        // Next stage (src/gpu_functions.cu:737-743, 800-806, 863-869).
-        if (s == 0) {
-            rx[1] = __builtin_fma(ax, dt2, vx0); ry[1] = __builtin_fma(ay, dt2, vy0);
-            rz[1] = __builtin_fma(az, dt2, vz0);
-            px = __builtin_fma(vx0, dt2, x0); py = __builtin_fma(vy0, dt2, y0);
-            pz = __builtin_fma(vz0, dt2, z0);
-            ux = rx[1]; uy = ry[1]; uz = rz[1];
-        } else if (s == 1) {
-            rx[2] = __builtin_fma(ax, dt2, vx0); ry[2] = __builtin_fma(ay, dt2, vy0);
-            rz[2] = __builtin_fma(az, dt2, vz0);
-            px = __builtin_fma(rx[1], dt2, x0); py = __builtin_fma(ry[1], dt2, y0);
-            pz = __builtin_fma(rz[1], dt2, z0);
-            ux = rx[2]; uy = ry[2]; uz = rz[2];
-        } else if (s == 2) {
-            rx[3] = __builtin_fma(ax, dt, vx0); ry[3] = __builtin_fma(ay, dt, vy0);
-            rz[3] = __builtin_fma(az, dt, vz0);
-            px = __builtin_fma(rx[2], dt, x0); py = __builtin_fma(ry[2], dt, y0);
-            pz = __builtin_fma(rz[2], dt, z0);
-            ux = rx[3]; uy = ry[3]; uz = rz[3];
+        S.a[wave][0][lane] = ax; S.a[wave][1][lane] = ay; S.a[wave][2][lane] = az;
+        S.flag[wave][lane] = hit ? 1 : 0;
+        __syncthreads();
+        bool any_hit = false;
+#pragma unroll
+        for (int w = 0; w < kDustSplit; ++w) any_hit = any_hit || (S.flag[w][lane] != 0);
+        gone = gone || (act && any_hit);
+        if (wave == 0 && valid) {
+            // total: wave partials in wave order (wave 0 holds the star and planets)
+            double tot[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double v = S.a[0][c][lane];
+                for (int w = 1; w < nparts; ++w) v += S.a[w][c][lane];
+                tot[c] = v;
+            }
+            if (ACCEL) {
+                A.ox[g] = gone ? 0.0 : tot[0];
+                A.oy[g] = gone ? 0.0 : tot[1];
+                A.oz[g] = gone ? 0.0 : tot[2];
+                A.removed[g] = gone ? 1 : 0;
+            } else if (s < 3) {
+                // next stage (src/gpu_functions.cu:737-743, 800-806, 863-869)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double x0 = S.x0[c][lane], v0 = S.v0[c][lane], uo = S.u[c][lane];
+                    if (s == 0) {
+                        S.k0[c][lane] = tot[c];
+                        S.p[c][lane] = __builtin_fma(v0, dt2, x0);
+                        S.u[c][lane] = __builtin_fma(tot[c], dt2, v0);
+                    } else if (s == 1) {
+                        const double r2v = __builtin_fma(tot[c], dt2, v0);
+                        S.k12[c][lane] = tot[c];
+                        S.p[c][lane] = __builtin_fma(uo, dt2, x0);
+                        S.r12[c][lane] = uo + r2v;
+                        S.u[c][lane] = r2v;
+                    } else {
+                        S.k12[c][lane] = S.k12[c][lane] + tot[c];
+                        S.p[c][lane] = __builtin_fma(uo, dt, x0);
+                        S.u[c][lane] = __builtin_fma(tot[c], dt, v0);
+                    }
+                }
+            } else if (live) {
+                // evolve (src/gpu_functions.cu:927-933); removal sets N = 0
+                if (gone) {
+                    A.N[t] = 0.0;
+                } else {
+                    const double dt6 = dt / 6.0;
+                    double* const X[3] = {A.x, A.y, A.z};
+                    double* const V[3] = {A.vx, A.vy, A.vz};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double x0 = S.x0[c][lane], v0 = S.v0[c][lane];
+                        X[c][t] = __builtin_fma(dt6, __builtin_fma(2.0, S.r12[c][lane], v0) + S.u[c][lane], x0);
+                        V[c][t] = __builtin_fma(dt6, __builtin_fma(2.0, S.k12[c][lane], S.k0[c][lane]) + tot[c], v0);
+                    }
+                }
+            }
         }
+        __syncthreads();
     }
-    if (g >= A.n_dust) return;
-    if (ACCEL) {
-        A.ox[g] = kx[0]; A.oy[g] = ky[0]; A.oz[g] = kz[0];
-        A.removed[g] = gone ? 1 : 0;
-        return;
-    }
-    if (!live) return;
-    if (gone) { A.N[t] = 0.0; return; }
-    // Evolve (src/gpu_functions.cu:927-933).
-    A.x[t] = __builtin_fma(dt6, __builtin_fma(2.0, rx[1] + rx[2], vx0) + rx[3], x0);
-    A.y[t] = __builtin_fma(dt6, __builtin_fma(2.0, ry[1] + ry[2], vy0) + ry[3], y0);
-    A.z[t] = __builtin_fma(dt6, __builtin_fma(2.0, rz[1] + rz[2], vz0) + rz[3], z0);
-    A.vx[t] = __builtin_fma(dt6, __builtin_fma(2.0, kx[1] + kx[2], kx[0]) + kx[3], vx0);
-    A.vy[t] = __builtin_fma(dt6, __builtin_fma(2.0, ky[1] + ky[2], ky[0]) + ky[3], vy0);
-    A.vz[t] = __builtin_fma(dt6, __builtin_fma(2.0, kz[1] + kz[2], kz[0]) + kz[3], vz0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1425,7 +1473,7 @@ int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel)
 {
     if (c->n_dust == 0) return DDYN_OK;
     TimedLaunch tl(c, DDYN_KCLASS_DUST);
-    const unsigned blocks = (unsigned)((c->n_dust + kDustWG - 1) / kDustWG);
+    const unsigned blocks = (unsigned)((c->n_dust + 63) / 64);
     const bool f32 = c->prm.precision == DDYN_FP32_FORCE;
     if (accel) {
         if (f32) k_dust<true, true><<<blocks, kDustWG, 0, c->stream>>>(A);

@@ -104,7 +104,8 @@ int ddyn_step(ddyn_ctx *ctx, double dt, int64_t nsteps);
 
 /* One force evaluation at the current state (stage-1 forces of the next step):
  * ax/ay/az get n_grav + n_dust accelerations [AU/yr^2]; removed (may be NULL)
- * gets n_dust flags (1 = the grain would be removed: sublimation or exact hit).
+ * gets n_dust flags (1 = the grain would be removed: sublimation or exact hit);
+ * a removed grain's acceleration is reported as zero.
  * No reference counterpart (the reference never exposes accelerations). */
 int ddyn_accel(ddyn_ctx *ctx, double *ax, double *ay, double *az, int32_t *removed);
 
