@@ -8,9 +8,10 @@ step (four force evaluations) of the whole system with the state resident in
 HBM.  Metric: directed pairwise interactions per second,
 n_grav (n_grav - 1) + n_dust n_grav per evaluation, four evaluations per step.
 
-The roofline entry is the massive swarm-pair kernel: 20 FLOP per directed
-interaction (FMA = 2, rsqrt = 1) over its mean launch time measured with HIP
-events on the library's stream, against the FP32 vector peak.  The CPU
+The roofline entry is the dominant kernel (the massive swarm-pair kernel, or
+the grain kernel where grains dominate): 20 FLOP per directed interaction
+(FMA = 2, rsqrt = 1) over its mean launch time measured with HIP events,
+against the FP32 vector peak.  The CPU
 baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample.
 
 Prints one JSON line.
@@ -59,17 +60,27 @@ def workload(name, precision):
     return w, desc
 
 
-def timed_run(w, precision, steps, warmup):
+def timed_run(w, precision, steps, warmup, in_order=False):
     """Warm up, time `steps` RK4 steps, then run `steps` more with a HIP event
-    pair around every launch on the library's stream.
+    pair around every launch.
 
-    The wall-clock figure comes from the first pass, which records no events:
-    an event pair per launch adds about 20 us per launch on this stack, a
-    large share of a step when launches are short.  The per-kernel means come
-    from the second pass.  Returns (seconds, kernel timings, initial-state
-    accelerations, |dE/E0| over the first pass)."""
+    The wall-clock figure comes from the first pass, which records no events
+    (an event pair per launch adds about 20 us per launch on this stack, a
+    large share of a step when launches are short) and uses the library's
+    default schedule (with grains: the grain kernel of one step beside the
+    massive stages of the next, on a second stream).  The per-kernel means
+    come from the second pass, which (where grains overlap the massive
+    stages) runs one launch per kernel in order, so that no kernel shares the
+    GPU with another while it is timed.  in_order runs both passes that way.
+    Returns (seconds, kernel timings, initial-state accelerations, |dE/E0|
+    over the first pass)."""
+    from dataclasses import replace
+    prm = w.params.with_precision(precision)
+    st = w.state
+    overlapped = st.n_dust > 0 and not (st.n_grav - prm.n_planets == 0 and st.n_grav <= 32)
+    timing_prm = replace(prm, schedule=1) if overlapped else prm
     with Engine(0) as eng:
-        eng.upload(w.state.copy(), w.params.with_precision(precision))
+        eng.upload(w.state.copy(), timing_prm if in_order else prm)
         a, _ = eng.accel()          # forces at the initial state, for the error check
         if warmup:
             eng.step(w.dt, warmup)
@@ -81,6 +92,7 @@ def timed_run(w, precision, steps, warmup):
         eng.synchronize()
         t1 = time.perf_counter()
         e1 = eng.energy()[0]
+        eng.set_params(timing_prm)
         eng.set_timing(True)
         eng.step(w.dt, steps)
         kt = {k: eng.read_timing(k) for k in (DDYN_KCLASS_PAIRS, DDYN_KCLASS_DUST)}
@@ -257,6 +269,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
     ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
+    ap.add_argument("--in-order", action="store_true",
+                    help="time every pass with one launch per kernel, in order (kernel profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -270,9 +284,9 @@ def main():
     w32, desc = workload(args.workload, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
 
-    t32, kt32, a32, drift32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup)
+    t32, kt32, a32, drift32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup, args.in_order)
     k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
-    t64, kt64, a64, drift64 = timed_run(w32, FP64, k64, min(args.warmup, 2))
+    t64, kt64, a64, drift64 = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ddyn.h"
@@ -246,6 +247,8 @@ struct StageArgs {
     double *Srx, *Sry, *Srz;     // v + 2 v2 + 2 v3 + v4, built left to right
     double *Sax, *Say, *Saz;     // a1 + 2 a2 + 2 a3 + a4, built left to right
     double *ox, *oy, *oz;        // accel output (STAGE == 4)
+    double4* n64;                // stage 3: where the next step's stage-0 positions go
+    float4* n32;
     double h;                    // dt/2, dt/2, dt for stages 0-2; dt/6 for stage 3
 };
 
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(kWG) void k_stage(StageArgs A)
         A.vx[i] = __builtin_fma(h, A.Sax[i] + ax, vx);
         A.vy[i] = __builtin_fma(h, A.Say[i] + ay, vy);
         A.vz[i] = __builtin_fma(h, A.Saz[i] + az, vz);
-        store_pos(A.p64[0], A.p32[0], i, nx, ny, nz, pi.w);
+        store_pos(A.n64, A.n32, i, nx, ny, nz, pi.w);
     }
 }
 
@@ -1258,6 +1261,11 @@ struct ddyn_ctx {
     // massive stage buffers, n_grav entries
     double4* p64[4] = {nullptr, nullptr, nullptr, nullptr};
     float4* p32[4] = {nullptr, nullptr, nullptr, nullptr};
+    // second set for the overlapped schedule (grains present): swapped with p64/p32 every step
+    double4* q64[4] = {nullptr, nullptr, nullptr, nullptr};
+    float4* q32[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t gstream = nullptr;   // grain kernel stream of the overlapped schedule
+    hipEvent_t ev_stage[2] = {nullptr, nullptr}, ev_dust[2] = {nullptr, nullptr};
     double* stage = nullptr;     // 9 * n_grav: vr, Sr, Sa
     double* part = nullptr;      // nslice * 3 * n_grav
     int nslice = 0, slice_len = 0, tpt = 1;
@@ -1272,7 +1280,8 @@ struct ddyn_ctx {
     bool timing = false;
     bool fuse_small = true;      // k_small_steps for star + planets only systems
     struct Ev { int cls; hipEvent_t a, b; };
-    std::vector<Ev> events;
+    std::vector<Ev> events;      // recorded, not yet read
+    std::vector<Ev> event_pool;  // read, reused by later launches
     int64_t t_launch[DDYN_KCLASS_COUNT] = {0, 0, 0};
     double t_ms[DDYN_KCLASS_COUNT] = {0, 0, 0};
 };
@@ -1307,7 +1316,7 @@ void free_state(ddyn_ctx* c)
 {
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
     dfree(c->mu); dfree(c->N); dfree(c->id);
-    for (int s = 0; s < 4; ++s) { dfree(c->p64[s]); dfree(c->p32[s]); }
+    for (int s = 0; s < 4; ++s) { dfree(c->p64[s]); dfree(c->p32[s]); dfree(c->q64[s]); dfree(c->q32[s]); }
     dfree(c->stage); dfree(c->part); dfree(c->acc); dfree(c->rem);
     dfree(c->beta_p); dfree(c->betasw_p);
     c->use_beta_p = false; c->have_betasw_p = false;
@@ -1325,20 +1334,33 @@ int dalloc(ddyn_ctx* c, T*& p, size_t count)
     return DDYN_OK;
 }
 
+// HIP event pair around one launch on stream `st` (kernel timing, when on).
 struct TimedLaunch {
     ddyn_ctx* c;
     int cls;
+    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    TimedLaunch(ddyn_ctx* c_, int cls_) : c(c_), cls(cls_)
+    TimedLaunch(ddyn_ctx* c_, int cls_, hipStream_t st_ = nullptr) : c(c_), cls(cls_), st(st_ ? st_ : c_->stream)
     {
         if (!c->timing) return;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
-        (void)hipEventRecord(a, c->stream);
+        if (!c->event_pool.empty()) {
+            a = c->event_pool.back().a;
+            b = c->event_pool.back().b;
+            c->event_pool.pop_back();
+        } else if (hipEventCreate(&a) != hipSuccess) {
+            a = nullptr;
+            return;
+        } else if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            a = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, st);
     }
     ~TimedLaunch()
     {
         if (!a) return;
-        (void)hipEventRecord(b, c->stream);
+        (void)hipEventRecord(b, st);
         c->events.push_back({cls, a, b});
     }
 };
@@ -1351,10 +1373,19 @@ void drain_events(ddyn_ctx* c)
             c->t_ms[e.cls] += ms;
             c->t_launch[e.cls] += 1;
         }
+        c->event_pool.push_back(e);
+    }
+    c->events.clear();
+}
+
+void destroy_events(ddyn_ctx* c)
+{
+    drain_events(c);
+    for (auto& e : c->event_pool) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
-    c->events.clear();
+    c->event_pool.clear();
 }
 
 int n_swarm(const ddyn_ctx* c) { return (int)(c->n_grav - c->prm.n_planets); }
@@ -1427,6 +1458,7 @@ StageArgs stage_args(ddyn_ctx* c, double h)
     A.vrx = c->stage; A.vry = c->stage + n; A.vrz = c->stage + 2 * n;
     A.Srx = c->stage + 3 * n; A.Sry = c->stage + 4 * n; A.Srz = c->stage + 5 * n;
     A.Sax = c->stage + 6 * n; A.Say = c->stage + 7 * n; A.Saz = c->stage + 8 * n;
+    A.n64 = c->p64[0]; A.n32 = c->p32[0];
     A.h = h;
     return A;
 }
@@ -1469,18 +1501,19 @@ DustArgs dust_args(ddyn_ctx* c, double dt)
     return A;
 }
 
-int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel)
+int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel, hipStream_t st = nullptr)
 {
     if (c->n_dust == 0) return DDYN_OK;
-    TimedLaunch tl(c, DDYN_KCLASS_DUST);
+    if (!st) st = c->stream;
+    TimedLaunch tl(c, DDYN_KCLASS_DUST, st);
     const unsigned blocks = (unsigned)((c->n_dust + 63) / 64);
     const bool f32 = c->prm.precision == DDYN_FP32_FORCE;
     if (accel) {
-        if (f32) k_dust<true, true><<<blocks, kDustWG, 0, c->stream>>>(A);
-        else k_dust<false, true><<<blocks, kDustWG, 0, c->stream>>>(A);
+        if (f32) k_dust<true, true><<<blocks, kDustWG, 0, st>>>(A);
+        else k_dust<false, true><<<blocks, kDustWG, 0, st>>>(A);
     } else {
-        if (f32) k_dust<true, false><<<blocks, kDustWG, 0, c->stream>>>(A);
-        else k_dust<false, false><<<blocks, kDustWG, 0, c->stream>>>(A);
+        if (f32) k_dust<true, false><<<blocks, kDustWG, 0, st>>>(A);
+        else k_dust<false, false><<<blocks, kDustWG, 0, st>>>(A);
     }
     DDYN_HIP(c, hipGetLastError());
     return DDYN_OK;
@@ -1497,6 +1530,38 @@ int one_step(ddyn_ctx* c, double dt)
     if ((rc = launch_pairs(c, 3))) return rc;
     if ((rc = launch_dust(c, dust_args(c, dt), false))) return rc;   // before the final update
     return launch_stage(c, 3, stage_args(c, dt / 6.0));
+}
+
+// Step k of a run of steps with the grain kernel on its own stream, so that it
+// overlaps the massive stages of step k+1 (same kernels and arguments as
+// one_step, so the results are identical).  The massive stage positions are
+// double buffered by step parity: the final massive stage writes the next
+// step's stage-0 positions into the other set, after the grain kernel of step
+// k-1 (the last reader of that set) has finished.  The caller joins the grain
+// stream after the last step.
+int overlapped_step(ddyn_ctx* c, double dt, int64_t k)
+{
+    int rc;
+    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
+    for (int s = 0; s < 3; ++s) {
+        if ((rc = launch_pairs(c, s))) return rc;
+        if ((rc = launch_stage(c, s, stage_args(c, hs[s])))) return rc;
+    }
+    DDYN_HIP(c, hipEventRecord(c->ev_stage[k & 1], c->stream));
+    if ((rc = launch_pairs(c, 3))) return rc;
+    DDYN_HIP(c, hipStreamWaitEvent(c->gstream, c->ev_stage[k & 1], 0));
+    if ((rc = launch_dust(c, dust_args(c, dt), false, c->gstream))) return rc;
+    DDYN_HIP(c, hipEventRecord(c->ev_dust[k & 1], c->gstream));
+    if (k > 0) DDYN_HIP(c, hipStreamWaitEvent(c->stream, c->ev_dust[(k - 1) & 1], 0));
+    StageArgs S = stage_args(c, dt / 6.0);
+    S.n64 = c->q64[0];
+    S.n32 = c->q32[0];
+    if ((rc = launch_stage(c, 3, S))) return rc;
+    for (int s = 0; s < 4; ++s) {
+        std::swap(c->p64[s], c->q64[s]);
+        std::swap(c->p32[s], c->q32[s]);
+    }
+    return DDYN_OK;
 }
 
 }  // namespace
@@ -1517,6 +1582,14 @@ int ddyn_create(ddyn_ctx** out, int device)
         delete c;
         return DDYN_ERR_HIP;
     }
+    bool ok = hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking) == hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k)
+        ok = hipEventCreateWithFlags(&c->ev_stage[k], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_dust[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        ddyn_destroy(c);
+        return DDYN_ERR_HIP;
+    }
     if (dalloc(c, c->beta_t, kMaxBins) || dalloc(c, c->betasw_t, kMaxBins) || dalloc(c, c->q_t, kMaxBins)) {
         ddyn_destroy(c);
         return DDYN_ERR_NOMEM;
@@ -1530,9 +1603,15 @@ void ddyn_destroy(ddyn_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    drain_events(c);
+    if (c->gstream) (void)hipStreamSynchronize(c->gstream);
+    destroy_events(c);
     free_state(c);
     dfree(c->beta_t); dfree(c->betasw_t); dfree(c->q_t);
+    for (int k = 0; k < 2; ++k) {
+        if (c->ev_stage[k]) (void)hipEventDestroy(c->ev_stage[k]);
+        if (c->ev_dust[k]) (void)hipEventDestroy(c->ev_dust[k]);
+    }
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1585,6 +1664,9 @@ int ddyn_upload_state(ddyn_ctx* c, int64_t n_grav, int64_t n_dust, const double*
     if (dalloc(c, c->id, na)) { free_state(c); return DDYN_ERR_NOMEM; }
     for (int s = 0; s < 4; ++s)
         if (dalloc(c, c->p64[s], ng) || dalloc(c, c->p32[s], ng)) { free_state(c); return DDYN_ERR_NOMEM; }
+    if (n_dust > 0)
+        for (int s = 0; s < 4; ++s)
+            if (dalloc(c, c->q64[s], ng) || dalloc(c, c->q32[s], ng)) { free_state(c); return DDYN_ERR_NOMEM; }
     if (dalloc(c, c->stage, 9 * ng) || dalloc(c, c->acc, 3 * na) ||
         dalloc(c, c->rem, (size_t)(n_dust > 0 ? n_dust : 1)) || dalloc(c, c->beta_p, (size_t)(n_dust > 0 ? n_dust : 1)) ||
         dalloc(c, c->betasw_p, (size_t)(n_dust > 0 ? n_dust : 1))) {
@@ -1660,6 +1742,12 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
             k_small_steps<<<blocks, kSmallWG, 0, c->stream>>>(A, n);
             DDYN_HIP(c, hipGetLastError());
         }
+        return DDYN_OK;
+    }
+    if (c->n_dust > 0 && c->fuse_small) {
+        for (int64_t k = 0; k < nsteps; ++k)
+            if ((rc = overlapped_step(c, dt, k))) return rc;
+        if (nsteps > 0) DDYN_HIP(c, hipStreamWaitEvent(c->stream, c->ev_dust[(nsteps - 1) & 1], 0));
         return DDYN_OK;
     }
     for (int64_t k = 0; k < nsteps; ++k)

@@ -31,7 +31,7 @@ class Params:
     Rstar: float = 0.0             # stellar radius [AU]
     light: float = LIGHT_AU        # speed of light [AU/yr]
     fp64_radius: float = 0.0       # FP32_FORCE: FP64 pair radius [AU] (0 -> 10*eps)
-    schedule: int = 0              # 0: fused multi-step launches where possible; 1: one per kernel
+    schedule: int = 0              # 0: fused multi-step launches, overlapped grain stream; 1: one per kernel, in order
 
     def with_precision(self, precision):
         return replace(self, precision=precision)

@@ -62,7 +62,9 @@ typedef struct {
     double  fp64_radius; /* DDYN_FP32_FORCE: swarm pairs closer than this [AU] are evaluated in
                             FP64; <= 0 selects 10*eps */
     int32_t schedule;    /* 0: fuse many steps per launch where the system allows it (star +
-                            planets only, <= 32 massive bodies); 1: one launch per kernel.
+                            planets only, <= 32 massive bodies), and with grains run the grain
+                            kernel of step k beside the massive stages of step k+1 on a second
+                            stream; 1: one launch per kernel, in order, on one stream.
                             Both give identical results. */
     int32_t reserved;    /* must be 0 */
 } ddyn_params;
