@@ -423,6 +423,7 @@ struct DustShared {
     double p[3][64], u[3][64];       // current stage position / velocity
     double k0[3][64], k12[3][64];    // k1 and k2 + k3 (accelerations)
     double r12[3][64];               // r2 + r3 (stage velocities)
+    double coef[4][64];              // mass, beta, beta_sw, charge-to-mass factor
     double a[kDustSplit][3][64];     // per-wave partial accelerations
     int32_t flag[kDustSplit][64];    // per-wave "grain removed" flags
 };
@@ -455,6 +456,20 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(DDYN_DU
             S.x0[c][lane] = xs[c]; S.v0[c][lane] = vs[c];
             S.p[c][lane] = xs[c]; S.u[c][lane] = vs[c];
         }
+        double m = 1.0, bet = 0.0, betsw = 0.0, qe = 0.0;
+        if (valid) {
+            m = A.mu[t];
+            const int32_t id = A.id[t];
+            if (A.beta_p) {
+                bet = A.beta_p[g];
+                betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+            } else if (id >= 0 && id < A.n_bins) {
+                bet = (double)A.beta_t[id];
+                betsw = (double)A.betasw_t[id];
+            }
+            qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
+        }
+        S.coef[0][lane] = m; S.coef[1][lane] = bet; S.coef[2][lane] = betsw; S.coef[3][lane] = qe;
     }
     __syncthreads();
     bool gone = false;
@@ -467,18 +482,8 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(DDYN_DU
         bool hit = false;
         if (wave == 0 && act) {
             // star and planets (src/gpu_functions.cu:679-730), FP64
-            double m = A.mu[t], bet = 0.0, betsw = 0.0, qe = 0.0;
-            const int32_t id = A.id[t];
-            if (A.beta_p) {
-                bet = A.beta_p[g];
-                betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
-            } else if (id >= 0 && id < A.n_bins) {
-                bet = (double)A.beta_t[id];
-                betsw = (double)A.betasw_t[id];
-            }
-            qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
-            hit = star_term(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane], bet, betsw,
-                            qe, m, ax, ay, az);
+            hit = star_term(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane],
+                            S.coef[1][lane], S.coef[2][lane], S.coef[3][lane], S.coef[0][lane], ax, ay, az);
             grain_fp64(A.p64[s], 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
         }
         if (__builtin_amdgcn_readfirstlane(__any(act)) && jb < je) {
@@ -577,9 +582,13 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(DDYN_DU
                     }
                 }
             } else if (live) {
-                // evolve (src/gpu_functions.cu:927-933); removal sets N = 0
+                // evolve (src/gpu_functions.cu:927-933); removal sets N = 0.  The
+                // empty asm makes the index opaque so that the compiler forms the
+                // six addresses here instead of holding them through the loop.
+                int64_t ti = t;
+                asm volatile("" : "+v"(ti));
                 if (gone) {
-                    A.N[t] = 0.0;
+                    A.N[ti] = 0.0;
                 } else {
                     const double dt6 = dt / 6.0;
                     double* const X[3] = {A.x, A.y, A.z};
@@ -587,8 +596,8 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(DDYN_DU
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const double x0 = S.x0[c][lane], v0 = S.v0[c][lane];
-                        X[c][t] = __builtin_fma(dt6, __builtin_fma(2.0, S.r12[c][lane], v0) + S.u[c][lane], x0);
-                        V[c][t] = __builtin_fma(dt6, __builtin_fma(2.0, S.k12[c][lane], S.k0[c][lane]) + tot[c], v0);
+                        X[c][ti] = __builtin_fma(dt6, __builtin_fma(2.0, S.r12[c][lane], v0) + S.u[c][lane], x0);
+                        V[c][ti] = __builtin_fma(dt6, __builtin_fma(2.0, S.k12[c][lane], S.k0[c][lane]) + tot[c], v0);
                     }
                 }
             }
