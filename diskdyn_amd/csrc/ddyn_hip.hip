@@ -1805,10 +1805,16 @@ int ddyn_download_state(ddyn_ctx* c, double* x, double* y, double* z, double* vx
 
 namespace {
 
-struct DevBuf {
-    double* p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+// Device buffer freed when it goes out of scope.
+extern "C++" {
+template <class T>
+struct DevBufT {
+    T* p = nullptr;
+    ~DevBufT() { if (p) (void)hipFree(p); }
 };
+}
+using DevBuf = DevBufT<double>;
+using DevBuf4 = DevBufT<double4>;
 
 }  // namespace
 
@@ -1831,7 +1837,9 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
     int vlen = (int)(((ng + vsl - 1) / vsl + kTile - 1) / kTile) * kTile;
     vsl = (int)((ng + vlen - 1) / vlen);
     DevBuf ckb, segb, finb, rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart;
+    DevBuf4 stgb;   // massive positions of stages 1-3 of every step of the current segment
     if (dalloc(c, ckb.p, (size_t)std::max<int64_t>(nseg, 1) * 7 * na) || dalloc(c, segb.p, (size_t)ck * 7 * na) ||
+        dalloc(c, stgb.p, (size_t)ck * 3 * ng) ||
         dalloc(c, finb.p, 7 * na) || dalloc(c, rbb.p, 3 * na) || dalloc(c, vbb.p, 3 * na) ||
         dalloc(c, Vbm.p, 12 * ng) || dalloc(c, Abm.p, 12 * ng) || dalloc(c, Rbg.p, 12 * ng) ||
         dalloc(c, mug.p, ng) || dalloc(c, gmu.p, ng) || dalloc(c, gbeta.p, std::max<size_t>(nd, 1)) ||
@@ -1872,19 +1880,31 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
     A.wpart = wpart.p; A.part = vpart.p; A.g_mu = gmu.p; A.g_beta = gbeta.p;
     const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
 
+    // The segment's forward recomputation keeps, per step, the state and the
+    // massive positions of stages 1-3 (written straight into stgb by pointing
+    // the stage buffers there), so the reverse sweep recomputes nothing.
+    double4* const own64[3] = {c->p64[1], c->p64[2], c->p64[3]};
+    auto stage_slot = [&](int64_t kk, int st) { return stgb.p + ((size_t)kk * 3 + (size_t)(st - 1)) * ng; };
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t k0 = s * ck, k1 = std::min(k0 + ck, nsteps);
         restore(ckb.p + (size_t)s * 7 * na);
         for (int64_t k = k0; k < k1; ++k) {
             save(segb.p + (size_t)(k - k0) * 7 * na);
-            if (k + 1 < k1 && (rc = one_step(c, dt))) return rc;
+            for (int st = 1; st <= 3; ++st) c->p64[st] = stage_slot(k - k0, st);
+            for (int st = 0; st < 3; ++st) {           // massive stages 0-2 of step k
+                if ((rc = launch_pairs(c, st))) break;
+                if ((rc = launch_stage(c, st, stage_args(c, hs[st])))) break;
+            }
+            if (!rc && k + 1 < k1) {                   // the rest of the step
+                if (!(rc = launch_pairs(c, 3)) && !(rc = launch_dust(c, dust_args(c, dt), false)))
+                    rc = launch_stage(c, 3, stage_args(c, dt / 6.0));
+            }
+            for (int st = 1; st <= 3; ++st) c->p64[st] = own64[st - 1];
+            if (rc) return rc;
         }
         for (int64_t k = k1 - 1; k >= k0; --k) {
             restore(segb.p + (size_t)(k - k0) * 7 * na);
-            for (int st = 0; st < 3; ++st) {           // massive stage positions of step k
-                if ((rc = launch_pairs(c, st))) return rc;
-                if ((rc = launch_stage(c, st, stage_args(c, hs[st])))) return rc;
-            }
+            for (int st = 1; st <= 3; ++st) A.p64[st] = stage_slot(k - k0, st);
             k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
             if (nd) {
                 k_adj_dust<<<(unsigned)((nd + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
@@ -1895,11 +1915,11 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
                 if (c->prm.precision == DDYN_FP32_FORCE) {
                     const double r64 = r64_of(c);
                     k_vjp_pairs32<<<dim3(bg, (unsigned)vsl), kWG, 0, c->stream>>>(
-                        c->p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c),
+                        A.p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c),
                         (float)(r64 * r64), vpart.p);
                 } else {
                     k_vjp_pairs<<<dim3(bg, (unsigned)vsl), kWG, 0, c->stream>>>(
-                        c->p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c),
+                        A.p64[st], Abm.p + (size_t)st * 3 * ng, (int)ng, c->prm.n_planets, vlen, eps2_of(c),
                         vpart.p);
                 }
                 switch (st) {
