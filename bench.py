@@ -28,12 +28,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from diskdyn_amd import synth  # noqa: E402
-from diskdyn_amd.engine import DDYN_KCLASS_DUST, DDYN_KCLASS_PAIRS, Engine  # noqa: E402
+from diskdyn_amd.engine import DDYN_KCLASS_DUST, DDYN_KCLASS_PAIRS, DDYN_KCLASS_STAGE, Engine  # noqa: E402
 from diskdyn_amd.system import FP32_FORCE, FP64  # noqa: E402
 
 FLOP_PER_INTERACTION = 20
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+# Algorithmic HBM bytes of the four massive stage epilogues (k_stage<0..3>) per
+# massive body per RK4 step: 200 + 272 + 248 + 224 B of state, stage positions
+# (FP64 + FP32 shadow) and RK4 sums, plus 3 FP64 slice partials read per stage.
+STAGE_BYTES_PER_BODY = 944
+STAGE_PARTIAL_BYTES_PER_SLICE = 4 * 3 * 8
 
 
 def workload(name, precision):
@@ -95,9 +101,25 @@ def timed_run(w, precision, steps, warmup, in_order=False):
         eng.set_params(timing_prm)
         eng.set_timing(True)
         eng.step(w.dt, steps)
-        kt = {k: eng.read_timing(k) for k in (DDYN_KCLASS_PAIRS, DDYN_KCLASS_DUST)}
+        kt = {k: eng.read_timing(k) for k in (DDYN_KCLASS_PAIRS, DDYN_KCLASS_STAGE, DDYN_KCLASS_DUST)}
         eng.set_timing(False)
+        kt["nslice"] = eng.pair_plan()[0]
     return t1 - t0, kt, a, abs((e1 - e0) / e0)
+
+
+def state_update(w, kt, steps):
+    """Achieved HBM bandwidth of the massive stage epilogues (the FP64 state
+    update) from their HIP-event times, or None where they do not run."""
+    nl, ms = kt[DDYN_KCLASS_STAGE]
+    if nl == 0:
+        return None
+    per_body = STAGE_BYTES_PER_BODY + STAGE_PARTIAL_BYTES_PER_SLICE * kt["nslice"]
+    nbytes = w.state.n_grav * per_body
+    ms_step = ms / steps
+    gbs = nbytes / (ms_step * 1e-3) / 1e9
+    return {"kernel": "k_stage<0..3>", "bytes_per_step": nbytes, "bytes_per_body": per_body,
+            "nslice": kt["nslice"], "ms_per_step": ms_step, "achieved_GBs": gbs,
+            "frac_hbm_peak": gbs / PEAK_HBM_GBS}
 
 
 def force_error(w, a_gpu, nsample=2048, seed=7):
@@ -341,6 +363,7 @@ def main():
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(kname, w32.name), "launches": nl, "avg_launch_ms": avg_ms,
                      "flop_per_launch": FLOP_PER_INTERACTION * inter_launch},
+        "state_update": state_update(w32, kt32, args.steps),
     }
     if not args.no_cpu:
         cb = cpu_baseline(w32)

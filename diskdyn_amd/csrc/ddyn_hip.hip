@@ -1995,6 +1995,15 @@ int ddyn_set_timing(ddyn_ctx* c, int enable)
     return DDYN_OK;
 }
 
+int ddyn_pair_plan(ddyn_ctx* c, int32_t* nslice, int32_t* slice_len)
+{
+    const int rc = check_ready(c);
+    if (rc) return rc;
+    if (nslice) *nslice = c->nslice;
+    if (slice_len) *slice_len = c->slice_len;
+    return DDYN_OK;
+}
+
 int ddyn_read_timing(ddyn_ctx* c, int32_t kclass, int64_t* launches, double* total_ms)
 {
     if (!c) return DDYN_ERR_ARG;

@@ -49,6 +49,7 @@ SIGNATURES = {
     "ddyn_synchronize": (ctypes.c_int, [_VP]),
     "ddyn_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ddyn_read_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _I64, _D]),
+    "ddyn_pair_plan": (ctypes.c_int, [_VP, _I32, _I32]),
 }
 
 _lib = None
@@ -209,6 +210,13 @@ class Engine:
         self._check(self.lib.ddyn_read_timing(self.ctx, int(kclass), ctypes.byref(n), ctypes.byref(ms)),
                     "ddyn_read_timing")
         return n.value, ms.value
+
+    def pair_plan(self):
+        """(nslice, slice_len) of the massive pair kernels for the uploaded state."""
+        ns = ctypes.c_int32(0)
+        ln = ctypes.c_int32(0)
+        self._check(self.lib.ddyn_pair_plan(self.ctx, ctypes.byref(ns), ctypes.byref(ln)), "ddyn_pair_plan")
+        return ns.value, ln.value
 
 
 def run(workload, device=0, nsteps=None, precision=None):

@@ -152,6 +152,12 @@ int ddyn_set_timing(ddyn_ctx *ctx, int enable);
 /* Totals since timing was enabled (synchronises). */
 int ddyn_read_timing(ddyn_ctx *ctx, int32_t kclass, int64_t *launches, double *total_ms);
 
+/* The source-slice split of the massive pair kernels for the uploaded state:
+ * nslice slices of slice_len swarm sources (0 and 0 without a swarm).  Each
+ * slice adds 3 FP64 partials per massive body per stage.  Needs parameters
+ * and a state.  No reference counterpart (measurement support). */
+int ddyn_pair_plan(ddyn_ctx *ctx, int32_t *nslice, int32_t *slice_len);
+
 #ifdef __cplusplus
 }
 #endif
