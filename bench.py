@@ -78,15 +78,23 @@ def timed_run(w, precision, steps, warmup, in_order=False):
     come from the second pass, which (where grains overlap the massive
     stages) runs one launch per kernel in order, so that no kernel shares the
     GPU with another while it is timed.  in_order runs both passes that way.
-    Returns (seconds, kernel timings, initial-state accelerations, |dE/E0|
-    over the first pass)."""
+    Also times one upload (host to HBM) and one download of the whole state,
+    which the timed region excludes.  Returns (seconds, kernel timings,
+    initial-state accelerations, |dE/E0| over the first pass, (upload s,
+    download s))."""
     from dataclasses import replace
     prm = w.params.with_precision(precision)
     st = w.state
     overlapped = st.n_dust > 0 and not (st.n_grav - prm.n_planets == 0 and st.n_grav <= 32)
     timing_prm = replace(prm, schedule=1) if overlapped else prm
     with Engine(0) as eng:
-        eng.upload(w.state.copy(), timing_prm if in_order else prm)
+        host = w.state.copy()
+        eng.upload(host, timing_prm if in_order else prm)     # first upload: allocation + copy
+        eng.synchronize()
+        tu0 = time.perf_counter()
+        eng.upload(host, timing_prm if in_order else prm)
+        eng.synchronize()
+        t_up = time.perf_counter() - tu0
         a, _ = eng.accel()          # forces at the initial state, for the error check
         if warmup:
             eng.step(w.dt, warmup)
@@ -97,6 +105,9 @@ def timed_run(w, precision, steps, warmup, in_order=False):
         eng.step(w.dt, steps)
         eng.synchronize()
         t1 = time.perf_counter()
+        td0 = time.perf_counter()
+        eng.download(host)
+        t_down = time.perf_counter() - td0
         e1 = eng.energy()[0]
         eng.set_params(timing_prm)
         eng.set_timing(True)
@@ -104,7 +115,7 @@ def timed_run(w, precision, steps, warmup, in_order=False):
         kt = {k: eng.read_timing(k) for k in (DDYN_KCLASS_PAIRS, DDYN_KCLASS_STAGE, DDYN_KCLASS_DUST)}
         eng.set_timing(False)
         kt["nslice"] = eng.pair_plan()[0]
-    return t1 - t0, kt, a, abs((e1 - e0) / e0)
+    return t1 - t0, kt, a, abs((e1 - e0) / e0), (t_up, t_down)
 
 
 def state_update(w, kt, steps):
@@ -306,9 +317,9 @@ def main():
     w32, desc = workload(args.workload, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
 
-    t32, kt32, a32, drift32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup, args.in_order)
+    t32, kt32, a32, drift32, io32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup, args.in_order)
     k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
-    t64, kt64, a64, drift64 = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
+    t64, kt64, a64, drift64, _ = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)
@@ -364,6 +375,10 @@ def main():
                      "traffic": measured_traffic(kname, w32.name), "launches": nl, "avg_launch_ms": avg_ms,
                      "flop_per_launch": FLOP_PER_INTERACTION * inter_launch},
         "state_update": state_update(w32, kt32, args.steps),
+        "host_transfer": {"upload_ms": 1e3 * io32[0], "download_ms": 1e3 * io32[1],
+                          "value_incl_transfer": per_step * args.steps / (t32 + io32[0] + io32[1]),
+                          "note": "one upload and one download of the whole state around the timed steps; "
+                                  "value excludes them (state resident in HBM)"},
     }
     if not args.no_cpu:
         cb = cpu_baseline(w32)

@@ -11,6 +11,8 @@ the massive G*M, beta (n_dust,) the per-grain radiation-pressure ratio.
 (src/read_in.cu:344-375) written in torch, so dL/d(m, a, e) follows by
 autograd from the adjoint's dL/d(initial state) and dL/dmu.
 """
+import time
+
 import numpy as np
 import torch
 
@@ -41,38 +43,59 @@ class Runner:
                 st.betasw_p = np.zeros(st.n_dust)
         return st
 
+    @property
+    def windowed(self):
+        """True when the reverse sweep is chained window by window (report set)."""
+        return self.report is not None and 0 < self.ckpt_every < self.nsteps
+
     def forward(self, st):
+        return self.forward_with_checkpoints(st)[0]
+
+    def forward_with_checkpoints(self, st):
+        """Final state, and in windowed mode the host states at the window
+        starts (the adjoint's checkpoints, so that it need not run the forward
+        pass again); None otherwise."""
+        ckpts = None
         with Engine(self.device) as eng:
             eng.upload(st, self.params)
             if self.report is None or not self.ckpt_every:
                 eng.step(self.dt, self.nsteps)
             else:
+                ckpts = [] if self.windowed else None
+                t0 = time.perf_counter()
                 for k0 in range(0, self.nsteps, self.ckpt_every):
+                    if ckpts is not None:
+                        ckpts.append(eng.download(st.copy()))
                     eng.step(self.dt, min(self.ckpt_every, self.nsteps - k0))
                     eng.synchronize()
-                    self.report(f"forward step {min(k0 + self.ckpt_every, self.nsteps)}/{self.nsteps}")
+                    self.report(f"forward step {min(k0 + self.ckpt_every, self.nsteps)}/{self.nsteps} "
+                                f"({time.perf_counter() - t0:.1f} s)")
             out = eng.download(st.copy())
-        return out
+        return out, ckpts
 
-    def adjoint(self, st, lam_r, lam_v):
+    def adjoint(self, st, lam_r, lam_v, ckpts=None):
         """dL/d(initial r, v), dL/dmu and dL/dbeta for dL/d(final r, v) = lam_r, lam_v.
 
         With `report` set, the reverse sweep is chained segment by segment
         (ddyn_adjoint on each checkpoint window, the adjoint state passed from
-        one window to the previous) and `report` is called after each window."""
-        if self.report is None or not self.ckpt_every or self.ckpt_every >= self.nsteps:
+        one window to the previous) and `report` is called after each window.
+        `ckpts` are the window-start states from forward_with_checkpoints;
+        without them the forward pass runs again to make them."""
+        if not self.windowed:
             with Engine(self.device) as eng:
                 eng.upload(st, self.params)
                 return eng.adjoint(self.dt, self.nsteps, lam_r, lam_v, self.ckpt_every)
         ck = self.ckpt_every
         starts = list(range(0, self.nsteps, ck))
+        t0 = time.perf_counter()
         with Engine(self.device) as eng:
-            eng.upload(st, self.params)
-            ckpts = []
-            for k0 in starts:                      # forward pass, host checkpoints
-                ckpts.append(eng.download(st.copy()))
-                eng.step(self.dt, min(ck, self.nsteps - k0))
-            self.report(f"forward {self.nsteps} steps done")
+            if ckpts is None:
+                eng.upload(st, self.params)
+                ckpts = []
+                for k0 in starts:                      # forward pass, host checkpoints
+                    ckpts.append(eng.download(st.copy()))
+                    eng.step(self.dt, min(ck, self.nsteps - k0))
+                self.report(f"forward {self.nsteps} steps done ({time.perf_counter() - t0:.1f} s)")
             g_mu = np.zeros(st.n_grav)
             g_beta = np.zeros(st.n_dust)
             for i in range(len(starts) - 1, -1, -1):
@@ -81,7 +104,7 @@ class Runner:
                 lam_r, lam_v, gm, gb = eng.adjoint(self.dt, n, lam_r, lam_v, n)
                 g_mu += gm
                 g_beta += gb
-                self.report(f"adjoint window {len(starts) - i}/{len(starts)} done")
+                self.report(f"adjoint window {len(starts) - i}/{len(starts)} done ({time.perf_counter() - t0:.1f} s)")
         return lam_r, lam_v, g_mu, g_beta
 
 
@@ -89,9 +112,10 @@ class Trajectory(torch.autograd.Function):
     @staticmethod
     def forward(ctx, r0, v0, mu, beta, runner):
         st = runner.make_state(r0, v0, mu, beta)
-        out = runner.forward(st)
+        out, ckpts = runner.forward_with_checkpoints(st)
         ctx.runner = runner
         ctx.state = st
+        ctx.ckpts = ckpts
         rT = torch.from_numpy(out.pos()).to(r0.dtype)
         vT = torch.from_numpy(out.vel()).to(v0.dtype)
         return rT, vT
@@ -100,7 +124,7 @@ class Trajectory(torch.autograd.Function):
     def backward(ctx, g_rT, g_vT):
         lam_r = np.zeros(ctx.state.pos().shape) if g_rT is None else g_rT.detach().cpu().numpy()
         lam_v = np.zeros(ctx.state.pos().shape) if g_vT is None else g_vT.detach().cpu().numpy()
-        g_r, g_v, g_mu, g_beta = ctx.runner.adjoint(ctx.state, lam_r, lam_v)
+        g_r, g_v, g_mu, g_beta = ctx.runner.adjoint(ctx.state, lam_r, lam_v, ctx.ckpts)
         return (torch.from_numpy(g_r), torch.from_numpy(g_v), torch.from_numpy(g_mu),
                 torch.from_numpy(np.ascontiguousarray(g_beta)), None)
 

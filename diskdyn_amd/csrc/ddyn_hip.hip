@@ -1853,12 +1853,15 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
         k_restore_state<<<bn, kWG, 0, c->stream>>>(na, src, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N);
         k_init_pos<<<bg, kWG, 0, c->stream>>>((int)ng, c->x, c->y, c->z, c->mu, c->p64[0], c->p32[0]);
     };
-    // forward pass with checkpoints
-    for (int64_t k = 0; k < nsteps; ++k) {
+    // forward pass with checkpoints, up to the start of the last segment (that
+    // segment's recomputation below runs its steps to the end)
+    const int64_t klast = nseg > 0 ? (nseg - 1) * ck : 0;
+    for (int64_t k = 0; k < klast; ++k) {
         if (k % ck == 0) save(ckb.p + (size_t)(k / ck) * 7 * na);
         if ((rc = one_step(c, dt))) return rc;
     }
-    save(finb.p);
+    if (nseg > 0) save(ckb.p + (size_t)(nseg - 1) * 7 * na);
+    else save(finb.p);
     DDYN_HIP(c, hipGetLastError());
     DDYN_HIP(c, hipMemcpyAsync(rbb.p, lam_r, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
     DDYN_HIP(c, hipMemcpyAsync(vbb.p, lam_v, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1895,13 +1898,14 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
                 if ((rc = launch_pairs(c, st))) break;
                 if ((rc = launch_stage(c, st, stage_args(c, hs[st])))) break;
             }
-            if (!rc && k + 1 < k1) {                   // the rest of the step
+            if (!rc && (k + 1 < k1 || s == nseg - 1)) {   // the rest of the step
                 if (!(rc = launch_pairs(c, 3)) && !(rc = launch_dust(c, dust_args(c, dt), false)))
                     rc = launch_stage(c, 3, stage_args(c, dt / 6.0));
             }
             for (int st = 1; st <= 3; ++st) c->p64[st] = own64[st - 1];
             if (rc) return rc;
         }
+        if (s == nseg - 1) save(finb.p);              // the forward final state
         for (int64_t k = k1 - 1; k >= k0; --k) {
             restore(segb.p + (size_t)(k - k0) * 7 * na);
             for (int st = 1; st <= 3; ++st) A.p64[st] = stage_slot(k - k0, st);

@@ -80,6 +80,13 @@ def test_segment_chained_adjoint_equals_single_call():
     assert len(msgs) == 4   # forward + 3 windows
     for x, y in zip(a, b):
         assert _rel(np.asarray(y), np.asarray(x)) <= 1e-12
+    # window-start states kept from the forward run: no second forward pass
+    run = Runner(st, w.params, w.dt, 9, ckpt_every=4, report=msgs.append)
+    _, ckpts = run.forward_with_checkpoints(st.copy())
+    assert len(ckpts) == 3
+    c = run.adjoint(st.copy(), lam_r, lam_v, ckpts)
+    for x, y in zip(a, c):
+        assert _rel(np.asarray(y), np.asarray(x)) <= 1e-12
 
 
 def test_gpu_adjoint_directional_fd(engine_cls):
