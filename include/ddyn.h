@@ -19,10 +19,14 @@
  * the reference (n_grav = star + planets + swarm, src/grid.cu:696).
  *
  * Conventions: functions return DDYN_OK (0) or a negative DDYN_ERR_* code and
- * never exit the process; ddyn_last_error() gives the message.  The caller owns
- * every host array (copied in upload/download); the context owns all device
- * memory and one HIP stream.  A context is not thread safe.  ddyn_step() is
- * asynchronous on the context stream; download, accel and energy synchronise.
+ * never exit the process; ddyn_last_error() gives the message.  After a
+ * DDYN_ERR_HIP from ddyn_step or ddyn_adjoint the device state is undefined:
+ * upload a state again before stepping.  The caller owns every host array
+ * (copied in upload/download); the context owns all device memory and two HIP
+ * streams (the main stream, and a grain stream used by the default schedule
+ * of ddyn_step).  A context is not thread safe.  ddyn_step() is asynchronous
+ * on the main stream (which waits for the grain stream before the call
+ * returns); download, accel and energy synchronise.
  */
 #ifndef DDYN_H
 #define DDYN_H
@@ -140,10 +144,10 @@ int ddyn_adjoint(ddyn_ctx *ctx, double dt, int64_t nsteps, int64_t ckpt_every,
                  const double *lam_r, const double *lam_v, double *g_r, double *g_v,
                  double *g_mu, double *g_beta);
 
-/* Wait for all work on the context stream. */
+/* Wait for all work of the context (the main stream, which every call joins). */
 int ddyn_synchronize(ddyn_ctx *ctx);
 
-/* Kernel timing with HIP events on the context stream.  Classes: */
+/* Kernel timing with HIP events around each launch (on the stream it runs on).  Classes: */
 #define DDYN_KCLASS_PAIRS 0   /* massive swarm-pair kernel */
 #define DDYN_KCLASS_STAGE 1   /* massive stage epilogue (direct terms + RK4 update) */
 #define DDYN_KCLASS_DUST 2    /* fused four-stage grain kernel */
