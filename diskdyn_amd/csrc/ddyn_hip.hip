@@ -38,8 +38,10 @@ constexpr int kTile = 256;          // sources per LDS tile in the massive pair 
 constexpr int kDustSplit = 4;       // grain kernel: waves per workgroup, each over 1/4 of the swarm sources
 constexpr int kDustWG = 64 * kDustSplit;   // 64 grains per workgroup, one per lane
 constexpr int kDustTile = 256;      // grain kernel: sources per FP32 partial and close-pair test
-constexpr int kTargetWGs = 16384;   // pair-kernel workgroups wanted (64 per CU: latency hiding)
-constexpr int kMinSlice = 2048;     // fewest sources per slice (bounds the partials' traffic)
+constexpr int kTargetWGs = 65536;   // pair-kernel workgroups wanted per launch: short workgroups keep
+                                    // the launch's tail short (measured on the 65,537- and
+                                    // 1,048,577-body swarms, 16,384 to 65,536)
+constexpr int kMinSlice = 1024;     // fewest sources per slice (bounds the partials' traffic)
 constexpr double kLightAU = 63239.6717367;   // src/ddyn.cuh:22
 constexpr double kFar = 1.0e15;     // position of padding sources/targets [AU]
 constexpr float kTiny2 = 1.0e-20f;  // FP32 r^2 floor for unsoftened pairs (keeps self pairs finite)
