@@ -179,18 +179,16 @@ __global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos3
     const int jbeg = n_planets + blockIdx.y * slice_len;
     const int jend = min(jbeg + slice_len, n_grav);
 
+    // FP64 target positions are read only when a tile is redone in FP64
     float xf[TPT], yf[TPT], zf[TPT], e2f[TPT], thr[TPT];
-    double xi[TPT], yi[TPT], zi[TPT], e2d[TPT], ax[TPT], ay[TPT], az[TPT];
+    double ax[TPT], ay[TPT], az[TPT];
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
         const int i = wbase + lane + 64 * k;
         const bool ok = i < n_grav;
-        const double4 p = ok ? pos64[i] : make_double4(kFar, kFar, kFar, 0.0);
         const float4 q = ok ? pos32[i] : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
-        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
         xf[k] = q.x; yf[k] = q.y; zf[k] = q.z;
         const bool sw = i >= n_planets;
-        e2d[k] = sw ? eps2 : 0.0;
         e2f[k] = sw ? fmaxf(eps2f, kTiny2) : kTiny2;
         thr[k] = r64sq + e2f[k];
         ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0;
@@ -224,7 +222,17 @@ __global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos3
                 }
             }
         }
-        if (redo) tile_fp64<TPT>(pos64, j0, j0 + tl, xi, yi, zi, e2d, ax, ay, az);
+        if (redo) {
+            double xi[TPT], yi[TPT], zi[TPT], e2d[TPT];
+#pragma unroll
+            for (int k = 0; k < TPT; ++k) {
+                const int i = wbase + lane + 64 * k;
+                const double4 p = (i < n_grav) ? pos64[i] : make_double4(kFar, kFar, kFar, 0.0);
+                xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+                e2d[k] = (i >= n_planets) ? eps2 : 0.0;
+            }
+            tile_fp64<TPT>(pos64, j0, j0 + tl, xi, yi, zi, e2d, ax, ay, az);
+        }
     }
     const size_t n = (size_t)n_grav;
     double* px = part + (size_t)blockIdx.y * 3 * n;
