@@ -803,6 +803,20 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
 //   grains:  star term differentiated in closed form (gravity, radiation
 //            pressure, PR and wind drag), pulls mu_j K(R_j - R_t).
 
+// 1/sqrt(x) for x > 0 in FP64 (the adjoint kernels): the hardware estimate
+// v_rsq_f64 refined by two Newton steps y <- y + y (1 - x y^2) / 2, which
+// leaves a few ulp of error; cheaper than an IEEE square root and division.
+__device__ inline double rsqrt_fp64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = __builtin_fma(-x * y, y, 1.0);
+        y = __builtin_fma(0.5 * y, r, y);
+    }
+    return y;
+}
+
 // Massive VJP, all pairs: targets k (one per lane), sources j in slice y of
 // [0, n_grav).  Writes part[(y*4 + c) * n + k], c = x, y, z, mu.
 __global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ pos,
@@ -837,7 +851,7 @@ __global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ p
             const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
             const double s2 = dx * dx + dy * dy + dz * dz + ((j >= n_planets) ? e2k : 0.0);
             if (s2 > 0.0) {
-                const double inv = 1.0 / __builtin_sqrt(s2);
+                const double inv = rsqrt_fp64(s2);
                 const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
                 const double wx = pk.w * aj.x - pj.w * akx, wy = pk.w * aj.y - pj.w * aky,
                              wz = pk.w * aj.z - pj.w * akz;
@@ -867,7 +881,7 @@ __device__ inline void vjp_tile_fp64(const double4* __restrict__ pos, const doub
         const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
         const double s2 = dx * dx + dy * dy + dz * dz + ((j >= n_planets) ? e2k : 0.0);
         if (s2 > 0.0) {
-            const double inv = 1.0 / __builtin_sqrt(s2);
+            const double inv = rsqrt_fp64(s2);
             const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
             const double wx = pk.w * ajx - pj.w * akx, wy = pk.w * ajy - pj.w * aky, wz = pk.w * ajz - pj.w * akz;
             const double dw = dx * wx + dy * wy + dz * wz;
@@ -1127,7 +1141,7 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
             const double4 pj = P[j];
             const double q[3] = {pj.x - Rs[0], pj.y - Rs[1], pj.z - Rs[2]};
             const double s2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + ((j >= A.n_planets) ? A.eps2 : 0.0);
-            const double inv = 1.0 / __builtin_sqrt(s2), inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            const double inv = rsqrt_fp64(s2), inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
             const double qA = q[0] * Av[0] + q[1] * Av[1] + q[2] * Av[2];
             double c4[4];
             for (int c = 0; c < 3; ++c) {
