@@ -10,39 +10,15 @@ term, a wrong sign or index, a wrong RK4 weight or stage velocity) fails:
   * removal rules: sublimation inside Rmin, exact hit on a massive body.
 """
 import math
-import os
 
 import numpy as np
 import pytest
 
 from diskdyn_amd import orbits, synth
-from diskdyn_amd.system import FP64, DiskState, Params, make_state
-from diskdyn_amd.units import GRAV, LIGHT_AU
+from diskdyn_amd.system import FP64, Params, make_state
+from diskdyn_amd.units import GRAV
 from oracle import oracle as O
-
-GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "golden_system.npz")
-
-
-def golden_state():
-    g = np.load(GOLDEN, allow_pickle=False)
-    n_planets, n_grav = int(g["meta"][0]), int(g["meta"][1])
-    eps, vsw, Wz, Bstar, Rstar, Rmin, light = (float(v) for v in g["meta"][2:])
-    n = len(g["mu"])
-    st = make_state(n_grav, n - n_grav)
-    st.x, st.y, st.z = (np.ascontiguousarray(g["pos"][:, k]) for k in range(3))
-    st.vx, st.vy, st.vz = (np.ascontiguousarray(g["vel"][:, k]) for k in range(3))
-    st.mu = np.ascontiguousarray(g["mu"])
-    st.id = np.full(n, -1, dtype=np.int32)
-    st.id[n_grav:] = np.arange(n - n_grav, dtype=np.int32)
-    st.beta_p = np.ascontiguousarray(g["beta"])
-    st.betasw_p = np.ascontiguousarray(g["betasw"])
-    st.beta_t = np.zeros(n - n_grav, dtype=np.float32)
-    st.betasw_t = np.zeros(n - n_grav, dtype=np.float32)
-    st.q_t = g["q"].astype(np.float32)
-    prm = Params(n_planets=n_planets, SWQ=1, FLQ=1, eps=eps, Rmin=Rmin, vsw=vsw, Wz=Wz, Bstar=Bstar,
-                 Rstar=Rstar, light=light)
-    return st, prm, g
-
+from tests.paritylib import golden_state
 
 def test_golden_accelerations():
     st, prm, g = golden_state()
