@@ -307,7 +307,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU split is described in DESIGN.md)", file=sys.stderr)
+              "(the swarm would shard by target block, the grains by grain range)", file=sys.stderr)
         return 2
     if args.adjoint:
         if args.workload not in ("cfg4", "cfg5"):

@@ -8,11 +8,16 @@
 //     k_stage<s>              (stages 0-2) FP64 star/planet direct terms + slice
 //                             partials, then the RK4 stage update that writes the
 //                             next stage's positions (FP64 and an FP32 shadow)
-//   k_dust                    all four grain stages in registers (RK4calc_dust,
+//   k_dust                    all four grain stages (RK4calc_dust,
 //                             src/gpu_functions.cu:627-943) against the massive
-//                             stage positions; star terms in FP64, swarm tiles in LDS
+//                             stage positions; four waves per 64 grains, each over
+//                             a quarter of the swarm (scalar-loaded sources), star
+//                             and planet terms in FP64, RK4 bookkeeping in LDS
 //   k_stage<3>                massive RK4 combination (RK4calc_massive_final,
 //                             src/gpu_functions.cu:613-625) fused into stage 4.
+//
+// With grains, ddyn_step's default schedule runs k_dust of step k on a second
+// stream beside the massive stages of step k+1 (overlapped_step).
 //
 // The reference evaluates massive pairs on 32x32 tile pairs with Newton's third
 // law and CAS atomics (src/gpu_functions.cu:5-13, 101-217); here every target sums
@@ -426,8 +431,9 @@ typedef __attribute__((address_space(4))) const d4v cd4v;
 
 // Per-workgroup grain records.  Wave 0 owns the RK4 bookkeeping; the other
 // waves read the current stage position from here and post their partial
-// accelerations.  Keeping this state in LDS instead of registers keeps the
-// source loop at <= 64 VGPRs (8 waves per SIMD).
+// accelerations.  Keeping this state in LDS (19.5 KiB) instead of registers
+// lets the kernel run at 7 waves per SIMD (72 VGPRs) without scratch.
+constexpr int kDustWavesPerEU = 7;
 struct DustShared {
     double x0[3][64], v0[3][64];     // state at the start of the step
     double p[3][64], u[3][64];       // current stage position / velocity
@@ -439,10 +445,8 @@ struct DustShared {
 };
 
 template <bool F32, bool ACCEL>
-#ifndef DDYN_DUST_WPE
-#define DDYN_DUST_WPE 7
-#endif
-__global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(DDYN_DUST_WPE, 8))) void k_dust(DustArgs A)
+__global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWavesPerEU, 8))) void k_dust(
+    DustArgs A)
 {
     __shared__ DustShared S;
     const int lane = threadIdx.x & 63;

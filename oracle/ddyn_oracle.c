@@ -1,7 +1,7 @@
 /*
  * ddyn_oracle.c -- FP64 CPU restatement of DiskDyn's dynamical hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle for the HIP library in
  * diskdyn_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it, and never as the thing measured or shipped.
  *

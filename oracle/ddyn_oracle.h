@@ -1,6 +1,6 @@
 /*
  * ddyn_oracle.h -- FP64 CPU oracle for DiskDyn's dynamical hot path.
- * TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ * TEST INFRASTRUCTURE ONLY (the oracle of the tests, never the product).
  */
 #ifndef DDYN_ORACLE_H
 #define DDYN_ORACLE_H

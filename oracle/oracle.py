@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/ddyn_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product (diskdyn_amd) never
+bench.py's cpu_baseline leg, as the reference results.  The product (diskdyn_amd) never
 imports this module.
 """
 import ctypes
