@@ -306,7 +306,7 @@ def main():
                     help="time every pass with one launch per kernel, in order (kernel profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the swarm would shard by target block, the grains by grain range)", file=sys.stderr)
         return 2
     if args.adjoint:

@@ -55,7 +55,7 @@ const char* error_text(int code)
         case 45: return "too many disks";
         case 46: return "too many blobs";
         case 55: return "data file version differs from this version";
-        case 56: return "data file nR differs from this build";
+        case 56: return "data file nR differs from the compiled radial grid size";
         case 57: return "blob needs 'cloud' or 'collision'";
         case 58: return "blob needs fourteen values";
         default: return "error";

@@ -85,13 +85,13 @@ int main(int argc, char** argv)
         }
         for (int k = 0; k < cfg.n_disks; ++k)
             if (cfg.disks[k].Ngrav || cfg.disks[k].Nsmall) {
-                std::fprintf(stderr, "ddyn: disk particle generation is not part of this build; "
+                std::fprintf(stderr, "ddyn: disk particle generation is not supported; "
                                      "start from a data file with -m\n");
                 return 2;
             }
         for (int k = 0; k < cfg.n_blobs; ++k)
             if (cfg.blobs[k].Ngrav || cfg.blobs[k].Nsmall) {
-                std::fprintf(stderr, "ddyn: blob particle generation is not part of this build; "
+                std::fprintf(stderr, "ddyn: blob particle generation is not supported; "
                                      "start from a data file with -m\n");
                 return 2;
             }

@@ -90,4 +90,4 @@ def test_param_with_generated_particles_is_refused(tmp_path):
     p = tmp_path / "d.param"
     p.write_text(PARAM.replace("-3.5 0.1 -1.0 0 0", "-3.5 0.1 -1.0 0 100"))
     r = subprocess.run([DDYN, "-i", str(p), "--quiet"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "not part of this build" in r.stderr
+    assert r.returncode == 2 and "not supported" in r.stderr
