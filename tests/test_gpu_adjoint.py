@@ -155,7 +155,7 @@ def test_swarm_window_adjoint_vs_finite_differences():
     # on its own): directional derivative along dmu_j = mu_j u_j, u ~ N(0, 1)
     u = np.zeros(ng)
     u[1:] = rng.normal(size=ng - 1)
-    dirs = {"mu[0]": (np.eye(ng)[0] * float(mu[0]), 1e-6), "swarm": (u * mu.detach().numpy(), 1e-3)}
+    dirs = {"mu[0]": (np.eye(ng)[0] * float(mu.detach()[0]), 1e-6), "swarm": (u * mu.detach().numpy(), 1e-3)}
     for name, (d, h) in dirs.items():
         dmu = torch.from_numpy(d)
         with torch.no_grad():
