@@ -370,21 +370,38 @@ struct DustArgs {
     int32_t* removed;       // accel mode outputs (n_dust)
 };
 
+// A grain's step-invariant factors of the star term, each computed with the
+// same operations as inside the term, so hoisting them changes no bit:
+// gfac = 1 - beta - beta_sw; drag = beta/c + beta_sw/v_sw with wind drag, else
+// beta mu0 / c (src/gpu_functions.cu:691-699).
+struct GrainCoef {
+    double gfac, drag, qe, m;
+};
+
+__device__ inline GrainCoef grain_coef(const DustArgs& A, double bet, double betsw, double qe, double m,
+                                       double m0)
+{
+    GrainCoef k;
+    k.gfac = 1.0 - bet - betsw;
+    k.drag = (A.SWQ == 1) ? bet / A.light + betsw / A.vsw : bet * m0 / A.light;
+    k.qe = qe;
+    k.m = m;
+    return k;
+}
+
 // Star term with radiation pressure, PR and stellar-wind drag and the Lorentz
 // force (src/gpu_functions.cu:679-718).  Returns true if the grain sublimates.
 __device__ inline bool star_term(const DustArgs& A, const double4 s0, double px, double py,
-                                 double pz, double ux, double uy, double uz, double bet,
-                                 double betsw, double qe, double m, double& ax, double& ay,
-                                 double& az)
+                                 double pz, double ux, double uy, double uz, const GrainCoef& k,
+                                 double& ax, double& ay, double& az)
 {
     const double dx = px - s0.x, dy = py - s0.y, dz = pz - s0.z;
     const double RR = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
     const double RR3 = RR * __builtin_sqrt(RR);
     if (RR <= A.Rmin2) return true;
     const double m0 = s0.w;
-    const double Fg = (1.0 - bet - betsw) * m0 / RR3;
-    const double Fb1 = (A.SWQ == 1) ? m0 / RR * (bet / A.light + betsw / A.vsw)
-                                    : bet * m0 / A.light / RR;
+    const double Fg = k.gfac * m0 / RR3;
+    const double Fb1 = (A.SWQ == 1) ? m0 / RR * k.drag : k.drag / RR;
     const double Fb2 = __builtin_fma(dz, uz, __builtin_fma(dy, uy, dx * ux)) / RR;
     const double Fdx = Fb1 * __builtin_fma(Fb2, dx, ux);
     const double Fdy = Fb1 * __builtin_fma(Fb2, dy, uy);
@@ -392,8 +409,8 @@ __device__ inline bool star_term(const DustArgs& A, const double4 s0, double px,
     double Flx = 0.0, Fly = 0.0;
     if (A.FLQ == 1) {
         const double Bz = A.Bstar * A.Rstar * A.Rstar * A.Rstar / RR3;
-        Flx = qe * __builtin_fma(-A.Wz, dx, uy) * Bz / m;
-        Fly = -qe * __builtin_fma(A.Wz, dy, ux) * Bz / m;
+        Flx = k.qe * __builtin_fma(-A.Wz, dx, uy) * Bz / k.m;
+        Fly = -k.qe * __builtin_fma(A.Wz, dy, ux) * Bz / k.m;
     }
     ax = __builtin_fma(-Fg, dx, -Fdx) + Flx;
     ay = __builtin_fma(-Fg, dy, -Fdy) + Fly;
@@ -439,7 +456,7 @@ struct DustShared {
     double p[3][64], u[3][64];       // current stage position / velocity
     double k0[3][64], k12[3][64];    // k1 and k2 + k3 (accelerations)
     double r12[3][64];               // r2 + r3 (stage velocities)
-    double coef[4][64];              // mass, beta, beta_sw, charge-to-mass factor
+    double coef[4][64];              // GrainCoef: mass, 1 - beta - beta_sw, drag factor, charge factor
     double a[kDustSplit][3][64];     // per-wave partial accelerations
     int32_t flag[kDustSplit][64];    // per-wave "grain removed" flags
 };
@@ -483,7 +500,8 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
             }
             qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
         }
-        S.coef[0][lane] = m; S.coef[1][lane] = bet; S.coef[2][lane] = betsw; S.coef[3][lane] = qe;
+        const GrainCoef k = grain_coef(A, bet, betsw, qe, m, A.p64[0][0].w);
+        S.coef[0][lane] = k.m; S.coef[1][lane] = k.gfac; S.coef[2][lane] = k.drag; S.coef[3][lane] = k.qe;
     }
     __syncthreads();
     bool gone = false;
@@ -496,8 +514,9 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
         bool hit = false;
         if (wave == 0 && act) {
             // star and planets (src/gpu_functions.cu:679-730), FP64
-            hit = star_term(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane],
-                            S.coef[1][lane], S.coef[2][lane], S.coef[3][lane], S.coef[0][lane], ax, ay, az);
+            const GrainCoef k = {S.coef[1][lane], S.coef[2][lane], S.coef[3][lane], S.coef[0][lane]};
+            hit = star_term(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane], k, ax, ay,
+                            az);
             grain_fp64(A.p64[s], 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
         }
         if (__builtin_amdgcn_readfirstlane(__any(act)) && jb < je) {
@@ -667,6 +686,7 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
         }
         qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
     }
+    const GrainCoef kg = grain_coef(A, bet, betsw, qe, m, A.mu[0]);
     bool removed = false;
     for (int step = 0; step < nsteps; ++step) {
         if (tid < ng) P[0][tid] = make_double4(mx, my, mz, mmu);
@@ -717,7 +737,7 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
             bool gone = false;
             for (int s = 0; s < 4 && !gone; ++s) {
                 double ax = 0.0, ay = 0.0, az = 0.0;
-                gone = star_term(A, P[s][0], px, py, pz, ux, uy, uz, bet, betsw, qe, m, ax, ay, az);
+                gone = star_term(A, P[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
                 bool hit = false;
                 grain_fp64(P[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
                 gone = gone || hit;
@@ -1075,8 +1095,8 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (live && !gone) {
             const double4* P = A.p64[st];
-            gone = star_term(D, P[0], R[st][0], R[st][1], R[st][2], V[st][0], V[st][1], V[st][2], bet,
-                             betsw, 0.0, m, ax, ay, az);
+            gone = star_term(D, P[0], R[st][0], R[st][1], R[st][2], V[st][0], V[st][1], V[st][2],
+                             grain_coef(D, bet, betsw, 0.0, m, P[0].w), ax, ay, az);
             bool hit = false;
             grain_fp64(P, 1, A.n_planets, R[st][0], R[st][1], R[st][2], 0.0, ax, ay, az, hit);
             grain_fp64(P, A.n_planets, A.n_grav, R[st][0], R[st][1], R[st][2], A.eps2, ax, ay, az, hit);
