@@ -643,75 +643,84 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
 // Many steps in one launch for small massive sets (no swarm, n_grav <= 32)
 // ---------------------------------------------------------------------------
 // With only a star and a few planets the massive integration is a few hundred
-// FP64 operations per step, so every workgroup (one wave) integrates it
-// redundantly in LDS and advances its own 64 grains; nothing is exchanged
-// between workgroups and one launch covers many steps.  The arithmetic is that
-// of k_stage and k_dust (same expressions, same order), so results equal the
-// one-launch-per-kernel schedule bit for bit.
+// FP64 operations per step, so every workgroup integrates it redundantly and
+// advances its own 64 grains; nothing is exchanged between workgroups and one
+// launch covers many steps.  The launch is latency-bound (a few workgroups,
+// long FP64 square-root and division chains), so the workgroup is a two-wave
+// pipeline: wave 0 (one massive body per lane) runs one step ahead and, while
+// wave 1 advances the grains through step n against the four stage positions
+// in one LDS buffer, finishes step n's massive update and writes step n+1's
+// stage positions into the other buffer.  One barrier per step.  The
+// arithmetic is that of k_stage and k_dust (same expressions, same order), so
+// results equal the one-launch-per-kernel schedule bit for bit.
 constexpr int kSmallMax = 32;
-constexpr int kSmallWG = 64;
+constexpr int kSmallGrains = 64;                 // grains per workgroup (wave 1)
+constexpr int kSmallWG = 2 * 64;                 // wave 0 massive, wave 1 grains
 constexpr int kSmallChunk = 1024;   // steps per launch of k_small_steps
 
+// Pair sum of the star and planets on massive body i at one stage
+// (src/gpu_functions.cu:157-172, as in k_stage).
+__device__ inline void small_massive_accel(const double4* __restrict__ P, int i, int np, double& ax,
+                                           double& ay, double& az)
+{
+    const double4 pi = P[i];
+    ax = 0.0; ay = 0.0; az = 0.0;
+    for (int j = 0; j < np; ++j) {
+        const double4 pj = P[j];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        const double f = r2 * __builtin_sqrt(r2);
+        const double q = (f != 0.0) ? pj.w / f : 0.0;
+        ax = __builtin_fma(dx, q, ax);
+        ay = __builtin_fma(dy, q, ay);
+        az = __builtin_fma(dz, q, az);
+    }
+}
+
+// Within one wave, LDS stores are seen by the wave's later loads (its LDS
+// operations run in order); this keeps the compiler from moving them across.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// NP > 0 fixes the number of star + planets at compile time, so the pair loops
+// unroll and their independent square-root and division chains overlap (the
+// sums keep their order); NP = 0 takes it from A.
+template <int NP>
 __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps)
 {
-    __shared__ double4 P[4][kSmallMax];
-    const int tid = threadIdx.x;
-    const int ng = A.n_grav, np = A.n_planets;
+    __shared__ double4 P[2][4][kSmallMax];
+    const int lane = threadIdx.x & 63;
+    const bool mwave = threadIdx.x < 64;
+    const int ng = A.n_grav, np = NP > 0 ? NP : A.n_planets;
     const double dt = A.dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
     const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
-    // massive body tid (registers): base state, stage velocity and sums
+
+    // wave 0, lane i < ng: massive body i (base state, stage velocity and sums)
+    const bool body = mwave && lane < ng;
     double mx = 0, my = 0, mz = 0, mvx = 0, mvy = 0, mvz = 0, mmu = 0;
     double vrx = 0, vry = 0, vrz = 0, Srx = 0, Sry = 0, Srz = 0, Sax = 0, Say = 0, Saz = 0;
-    if (tid < ng) {
-        mx = A.x[tid]; my = A.y[tid]; mz = A.z[tid];
-        mvx = A.vx[tid]; mvy = A.vy[tid]; mvz = A.vz[tid]; mmu = A.mu[tid];
+    if (body) {
+        mx = A.x[lane]; my = A.y[lane]; mz = A.z[lane];
+        mvx = A.vx[lane]; mvy = A.vy[lane]; mvz = A.vz[lane]; mmu = A.mu[lane];
     }
-    // grain of this lane
-    const int g = blockIdx.x * kSmallWG + tid;
-    const int64_t t = (int64_t)ng + g;
-    bool live = false;
-    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0, m = 1, bet = 0, betsw = 0, qe = 0;
-    if (g < A.n_dust) {
-        live = A.N[t] != 0.0;
-        x0 = A.x[t]; y0 = A.y[t]; z0 = A.z[t];
-        vx0 = A.vx[t]; vy0 = A.vy[t]; vz0 = A.vz[t];
-        m = A.mu[t];
-        const int32_t id = A.id[t];
-        if (A.beta_p) {
-            bet = A.beta_p[g];
-            betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
-        } else if (id >= 0 && id < A.n_bins) {
-            bet = (double)A.beta_t[id];
-            betsw = (double)A.betasw_t[id];
-        }
-        qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
-    }
-    const GrainCoef kg = grain_coef(A, bet, betsw, qe, m, A.mu[0]);
-    bool removed = false;
-    for (int step = 0; step < nsteps; ++step) {
-        if (tid < ng) P[0][tid] = make_double4(mx, my, mz, mmu);
-        // massive stages 0-2 (k_stage<0..2>)
+    // massive stages 0-2 of a step (k_stage<0..2>) into P[buf]
+    auto massive_stages = [&](int buf) {
+        if (body) P[buf][0][lane] = make_double4(mx, my, mz, mmu);
+        wave_lds_sync();
         for (int s = 0; s < 3; ++s) {
-            __syncthreads();
-            if (tid < ng) {
-                const double4 pi = P[s][tid];
-                double ax = 0.0, ay = 0.0, az = 0.0;
-                for (int j = 0; j < np; ++j) {
-                    const double4 pj = P[s][j];
-                    const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                    const double f = r2 * __builtin_sqrt(r2);
-                    const double q = (f != 0.0) ? pj.w / f : 0.0;
-                    ax = __builtin_fma(dx, q, ax);
-                    ay = __builtin_fma(dy, q, ay);
-                    az = __builtin_fma(dz, q, az);
-                }
+            if (body) {
+                double ax, ay, az;
+                small_massive_accel(P[buf][s], lane, np, ax, ay, az);
                 const double h = hs[s];
                 const double ux = s ? vrx : mvx, uy = s ? vry : mvy, uz = s ? vrz : mvz;
                 const double rx = __builtin_fma(ax, h, mvx), ry = __builtin_fma(ay, h, mvy),
                              rz = __builtin_fma(az, h, mvz);
-                P[s + 1][tid] = make_double4(__builtin_fma(ux, h, mx), __builtin_fma(uy, h, my),
-                                             __builtin_fma(uz, h, mz), pi.w);
+                P[buf][s + 1][lane] = make_double4(__builtin_fma(ux, h, mx), __builtin_fma(uy, h, my),
+                                                   __builtin_fma(uz, h, mz), mmu);
                 if (s == 0) {
                     Srx = __builtin_fma(2.0, rx, mvx); Sry = __builtin_fma(2.0, ry, mvy);
                     Srz = __builtin_fma(2.0, rz, mvz);
@@ -728,18 +737,62 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
                 }
                 vrx = rx; vry = ry; vrz = rz;
             }
+            wave_lds_sync();
         }
-        __syncthreads();
-        // grains against the four stage positions (k_dust in the FP64 form)
-        if (live && !removed) {
+    };
+
+    // wave 1, lane l: grain blockIdx.x * 64 + l
+    const int g = blockIdx.x * kSmallGrains + lane;
+    const int64_t t = (int64_t)ng + g;
+    const bool mine = !mwave && g < A.n_dust;
+    bool live = false;
+    double x0 = 0, y0 = 0, z0 = 0, vx0 = 0, vy0 = 0, vz0 = 0, m = 1, bet = 0, betsw = 0, qe = 0;
+    if (mine) {
+        live = A.N[t] != 0.0;
+        x0 = A.x[t]; y0 = A.y[t]; z0 = A.z[t];
+        vx0 = A.vx[t]; vy0 = A.vy[t]; vz0 = A.vz[t];
+        m = A.mu[t];
+        const int32_t id = A.id[t];
+        if (A.beta_p) {
+            bet = A.beta_p[g];
+            betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+        } else if (id >= 0 && id < A.n_bins) {
+            bet = (double)A.beta_t[id];
+            betsw = (double)A.betasw_t[id];
+        }
+        qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
+    }
+    const GrainCoef kg = grain_coef(A, bet, betsw, qe, m, A.mu[0]);
+    bool removed = false;
+
+    if (mwave) massive_stages(0);
+    __syncthreads();
+    for (int step = 0; step < nsteps; ++step) {
+        const int buf = step & 1;
+        if (mwave) {
+            // massive stage 3 = final combination of this step (k_stage<3>) ...
+            if (body) {
+                double ax, ay, az;
+                small_massive_accel(P[buf][3], lane, np, ax, ay, az);
+                mx = __builtin_fma(dt6, Srx, mx); my = __builtin_fma(dt6, Sry, my);
+                mz = __builtin_fma(dt6, Srz, mz);
+                mvx = __builtin_fma(dt6, Sax + ax, mvx);
+                mvy = __builtin_fma(dt6, Say + ay, mvy);
+                mvz = __builtin_fma(dt6, Saz + az, mvz);
+            }
+            // ... and the stage positions of the next step into the other buffer
+            if (step + 1 < nsteps) massive_stages(buf ^ 1);
+        } else if (live && !removed) {
+            // grains against this step's four stage positions (k_dust in the FP64 form)
+            const double4(*Ps)[kSmallMax] = P[buf];
             double px = x0, py = y0, pz = z0, ux = vx0, uy = vy0, uz = vz0;
             double kx[4], ky[4], kz[4], rx[4], ry[4], rz[4];
             bool gone = false;
             for (int s = 0; s < 4 && !gone; ++s) {
                 double ax = 0.0, ay = 0.0, az = 0.0;
-                gone = star_term(A, P[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
+                gone = star_term(A, Ps[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
                 bool hit = false;
-                grain_fp64(P[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
+                grain_fp64(Ps[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
                 gone = gone || hit;
                 kx[s] = ax; ky[s] = ay; kz[s] = az;
                 if (s == 0) {
@@ -775,29 +828,8 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
             }
         }
         __syncthreads();
-        // massive stage 3 = final combination (k_stage<3>)
-        if (tid < ng) {
-            const double4 pi = P[3][tid];
-            double ax = 0.0, ay = 0.0, az = 0.0;
-            for (int j = 0; j < np; ++j) {
-                const double4 pj = P[3][j];
-                const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                const double f = r2 * __builtin_sqrt(r2);
-                const double q = (f != 0.0) ? pj.w / f : 0.0;
-                ax = __builtin_fma(dx, q, ax);
-                ay = __builtin_fma(dy, q, ay);
-                az = __builtin_fma(dz, q, az);
-            }
-            mx = __builtin_fma(dt6, Srx, mx); my = __builtin_fma(dt6, Sry, my);
-            mz = __builtin_fma(dt6, Srz, mz);
-            mvx = __builtin_fma(dt6, Sax + ax, mvx);
-            mvy = __builtin_fma(dt6, Say + ay, mvy);
-            mvz = __builtin_fma(dt6, Saz + az, mvz);
-        }
-        __syncthreads();
     }
-    if (g < A.n_dust && live) {
+    if (mine && live) {
         if (removed) {
             A.N[t] = 0.0;
         } else {
@@ -805,10 +837,10 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
             A.vx[t] = vx0; A.vy[t] = vy0; A.vz[t] = vz0;
         }
     }
-    if (blockIdx.x == 0 && tid < ng) {
-        A.x[tid] = mx; A.y[tid] = my; A.z[tid] = mz;
-        A.vx[tid] = mvx; A.vy[tid] = mvy; A.vz[tid] = mvz;
-        store_pos(const_cast<double4*>(A.p64[0]), const_cast<float4*>(A.p32[0]), tid, mx, my, mz, mmu);
+    if (blockIdx.x == 0 && body) {
+        A.x[lane] = mx; A.y[lane] = my; A.z[lane] = mz;
+        A.vx[lane] = mvx; A.vy[lane] = mvy; A.vz[lane] = mvz;
+        store_pos(const_cast<double4*>(A.p64[0]), const_cast<float4*>(A.p32[0]), lane, mx, my, mz, mmu);
     }
 }
 
@@ -1792,11 +1824,17 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
     if (n_swarm(c) == 0 && c->n_grav <= kSmallMax && c->fuse_small) {
         // star + planets only: many steps per launch (bounded launch length)
         const DustArgs A = dust_args(c, dt);
-        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + kSmallWG - 1) / kSmallWG);
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + kSmallGrains - 1) / kSmallGrains);
         for (int64_t k = 0; k < nsteps; k += kSmallChunk) {
             const int n = (int)std::min<int64_t>(kSmallChunk, nsteps - k);
             TimedLaunch tl(c, DDYN_KCLASS_DUST);
-            k_small_steps<<<blocks, kSmallWG, 0, c->stream>>>(A, n);
+            switch (A.n_planets) {
+            case 1: k_small_steps<1><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 2: k_small_steps<2><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 3: k_small_steps<3><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 4: k_small_steps<4><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            default: k_small_steps<0><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            }
             DDYN_HIP(c, hipGetLastError());
         }
         return DDYN_OK;

@@ -3,7 +3,7 @@ import os
 
 import numpy as np
 
-from diskdyn_amd.system import Params, make_state
+from diskdyn_amd.system import DiskState, Params, make_state
 
 
 def rel_err_rows(a, b, floor=0.0):
@@ -23,6 +23,16 @@ def vel_rel_err(st_a, st_b, sl=slice(None)):
     pa = np.stack([st_a.vx[sl], st_a.vy[sl], st_a.vz[sl]], axis=1)
     pb = np.stack([st_b.vx[sl], st_b.vy[sl], st_b.vz[sl]], axis=1)
     return rel_err_rows(pa, pb)
+
+
+def subset(st, grav, dust):
+    """State with the massive rows `grav` and the grain rows `dust` of st."""
+    rows = np.concatenate([np.asarray(grav, dtype=np.int64), st.n_grav + np.asarray(dust, dtype=np.int64)])
+    pick = lambda a: np.ascontiguousarray(a[rows])  # noqa: E731
+    pd = lambda a: None if a is None else np.ascontiguousarray(a[np.asarray(dust, dtype=np.int64)])  # noqa: E731
+    return DiskState(len(grav), len(dust), pick(st.x), pick(st.y), pick(st.z), pick(st.vx), pick(st.vy),
+                     pick(st.vz), pick(st.mu), pick(st.N), pick(st.id), pd(st.beta_p), pd(st.betasw_p),
+                     st.beta_t, st.betasw_t, st.q_t)
 
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden_system.npz")

@@ -12,21 +12,11 @@ import numpy as np
 import pytest
 
 from diskdyn_amd import synth
-from diskdyn_amd.system import FP32_FORCE, FP64, DiskState
+from diskdyn_amd.system import FP32_FORCE, FP64
 from oracle import oracle as O
-from tests.paritylib import pos_rel_err, rel_err_rows
+from tests.paritylib import pos_rel_err, rel_err_rows, subset
 
 pytestmark = pytest.mark.gpu
-
-
-def subset(st, grav, dust):
-    """State with the massive rows `grav` and the grain rows `dust` of st."""
-    rows = np.concatenate([np.asarray(grav, dtype=np.int64), st.n_grav + np.asarray(dust, dtype=np.int64)])
-    pick = lambda a: np.ascontiguousarray(a[rows])  # noqa: E731
-    pd = lambda a: None if a is None else np.ascontiguousarray(a[np.asarray(dust, dtype=np.int64)])  # noqa: E731
-    return DiskState(len(grav), len(dust), pick(st.x), pick(st.y), pick(st.z), pick(st.vx), pick(st.vy),
-                     pick(st.vz), pick(st.mu), pick(st.N), pick(st.id), pd(st.beta_p), pd(st.betasw_p),
-                     st.beta_t, st.betasw_t, st.q_t)
 
 
 def debris(n_swarm, n_dust, precision):

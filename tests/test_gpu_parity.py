@@ -4,13 +4,15 @@ Bars: plain FP64 form <= 1e-12 relative on per-particle accelerations and
 <= 1e-10 relative on positions after the stated steps; FP32-force form
 <= 1e-5 median relative acceleration error.
 """
+from dataclasses import replace
+
 import numpy as np
 import pytest
 
 from diskdyn_amd import synth
 from diskdyn_amd.system import FP32_FORCE, FP64
 from oracle import oracle as O
-from tests.paritylib import pos_rel_err, rel_err_rows, vel_rel_err
+from tests.paritylib import pos_rel_err, rel_err_rows, subset, vel_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -89,16 +91,36 @@ def test_step_fp32(engine_cls, name, nsteps):
     assert np.median(ep) <= 1e-8
 
 
-@pytest.mark.parametrize("name,nsteps", [("cfg1", 300), ("cfg4", 50)])
+def _planets_only(n_bodies):
+    """Star, planet and n_bodies - 2 former swarm bodies, all unsoftened
+    planets, with 700 ring grains under radiation pressure and PR drag."""
+    w = synth.config3(n_dust=700, n_swarm=max(n_bodies - 2, 1), precision=FP64)
+    st = subset(w.state, range(n_bodies), range(700))
+    return replace(w, state=st, params=replace(w.params, n_planets=n_bodies))
+
+
+FUSED = {
+    "cfg1": SMALL["cfg1"],
+    "cfg4": SMALL["cfg4"],
+    "star_only": lambda: replace(SMALL["cfg1"](), state=subset(synth.config1(n_dust=512).state, [0], range(512)),
+                                 params=replace(SMALL["cfg1"]().params, n_planets=1)),
+    "bodies4": lambda: _planets_only(4),
+    "bodies6": lambda: _planets_only(6),
+}
+
+
+@pytest.mark.parametrize("name,nsteps", [("cfg1", 300), ("cfg4", 50), ("star_only", 60), ("bodies4", 40),
+                                         ("bodies6", 40)])
 def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps):
-    """Star + planets only: the multi-step kernel equals one launch per kernel bit for bit."""
-    from dataclasses import replace
-    w = SMALL[name]()
+    """Star + planets only: the multi-step kernel equals one launch per kernel
+    bit for bit, for 1 to 6 massive bodies (the kernel is specialised for 1-4)."""
+    w = FUSED[name]()
     st = w.state.copy()
     # one grain inside Rmin, one exactly on a planet: removal must agree too
     g0 = st.n_grav
     st.x[g0], st.y[g0], st.z[g0] = st.x[0] + 0.01, st.y[0], st.z[0]
-    st.x[g0 + 1], st.y[g0 + 1], st.z[g0 + 1] = st.x[1], st.y[1], st.z[1]
+    if st.n_grav > 1:
+        st.x[g0 + 1], st.y[g0 + 1], st.z[g0 + 1] = st.x[1], st.y[1], st.z[1]
     prm = replace(w.params, Rmin=0.05)
     out = []
     for schedule in (0, 1):
@@ -122,7 +144,6 @@ def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision
     step k+1 (double-buffered stage positions) equals one launch per kernel in
     order, bit for bit, over two calls with grain removals, and the forces at
     the end state agree."""
-    from dataclasses import replace
     w = synth.config3(n_dust=3000, n_swarm=700, precision=precision)
     st = w.state.copy()
     g0 = st.n_grav
