@@ -318,8 +318,10 @@ def main():
     per_step = 4 * w32.interactions_per_eval()
 
     t32, kt32, a32, drift32, io32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup, args.in_order)
+    print(f"[bench] {args.workload} FP32-force: {args.steps} steps in {t32:.3f} s", file=sys.stderr, flush=True)
     k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
     t64, kt64, a64, drift64, _ = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
+    print(f"[bench] {args.workload} plain FP64: {k64} steps in {t64:.3f} s", file=sys.stderr, flush=True)
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)

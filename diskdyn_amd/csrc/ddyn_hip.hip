@@ -143,8 +143,7 @@ template <int TPT, bool SELF>
 __device__ inline void tile_sum32(const float4* __restrict__ tile, int self0, const float* xf, const float* yf,
                                   const float* zf, const float* e2f, float* fx, float* fy, float* fz, float* rmin)
 {
-#pragma unroll 4
-    for (int jj = 0; jj < kTile; ++jj) {
+    auto one = [&](int jj) {
         const float4 s = tile[jj];
 #pragma unroll
         for (int k = 0; k < TPT; ++k) {
@@ -161,6 +160,18 @@ __device__ inline void tile_sum32(const float4* __restrict__ tile, int self0, co
             fy[k] = __builtin_fmaf(dy, q, fy[k]);
             fz[k] = __builtin_fmaf(dz, q, fz[k]);
         }
+    };
+    // Eight sources per loop iteration in two groups of four that the
+    // scheduler may not merge: the pair law's transcendental and accumulation
+    // chains issue fastest in groups of four, and a longer loop body issues
+    // slower again.
+#pragma unroll 1
+    for (int jj = 0; jj < kTile; jj += 8) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) one(jj + u);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 4; u < 8; ++u) one(jj + u);
     }
 }
 
