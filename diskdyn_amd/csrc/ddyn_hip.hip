@@ -17,7 +17,10 @@
 //                             src/gpu_functions.cu:613-625) fused into stage 4.
 //
 // With grains, ddyn_step's default schedule runs k_dust of step k on a second
-// stream beside the massive stages of step k+1 (overlapped_step).
+// stream beside the massive stages of step k+1 (overlapped_step).  Without a
+// swarm and with at most 32 massive bodies (star and planets only), the default
+// schedule is k_small_steps instead: up to 1,024 whole steps per launch, each
+// workgroup integrating the massive set redundantly beside its 64 grains.
 //
 // The reference evaluates massive pairs on 32x32 tile pairs with Newton's third
 // law and CAS atomics (src/gpu_functions.cu:5-13, 101-217); here every target sums
