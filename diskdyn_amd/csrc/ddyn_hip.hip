@@ -382,6 +382,7 @@ struct DustArgs {
     int SWQ, FLQ;
     double *ox, *oy, *oz;   // accel mode outputs (n_dust)
     int32_t* removed;       // accel mode outputs (n_dust)
+    double* mfin;           // k_small_steps: final massive x, y, z, vx, vy, vz (n_grav each)
 };
 
 // A grain's step-invariant factors of the star term, each computed with the
@@ -851,11 +852,28 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
             A.vx[t] = vx0; A.vy[t] = vy0; A.vz[t] = vz0;
         }
     }
+    // Every workgroup read the massive state at its start, and a workgroup may
+    // start after workgroup 0 has finished (more workgroups than fit on the GPU
+    // at once), so the final massive state goes to mfin; k_small_commit, the
+    // next launch on the stream, moves it into the state.
     if (blockIdx.x == 0 && body) {
-        A.x[lane] = mx; A.y[lane] = my; A.z[lane] = mz;
-        A.vx[lane] = mvx; A.vy[lane] = mvy; A.vz[lane] = mvz;
-        store_pos(const_cast<double4*>(A.p64[0]), const_cast<float4*>(A.p32[0]), lane, mx, my, mz, mmu);
+        const int n = ng;
+        A.mfin[lane] = mx; A.mfin[n + lane] = my; A.mfin[2 * n + lane] = mz;
+        A.mfin[3 * n + lane] = mvx; A.mfin[4 * n + lane] = mvy; A.mfin[5 * n + lane] = mvz;
     }
+}
+
+// Moves the massive state k_small_steps left in mfin into the state arrays and
+// the stage-0 positions (one lane per body, n_grav <= kSmallMax).
+__global__ __launch_bounds__(64) void k_small_commit(DustArgs A)
+{
+    const int i = threadIdx.x;
+    const int n = A.n_grav;
+    if (i >= n) return;
+    const double x = A.mfin[i], y = A.mfin[n + i], z = A.mfin[2 * n + i];
+    A.x[i] = x; A.y[i] = y; A.z[i] = z;
+    A.vx[i] = A.mfin[3 * n + i]; A.vy[i] = A.mfin[4 * n + i]; A.vz[i] = A.mfin[5 * n + i];
+    store_pos(const_cast<double4*>(A.p64[0]), const_cast<float4*>(A.p32[0]), i, x, y, z, A.mu[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1837,7 +1855,8 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
     DDYN_HIP(c, hipSetDevice(c->device));
     if (n_swarm(c) == 0 && c->n_grav <= kSmallMax && c->fuse_small) {
         // star + planets only: many steps per launch (bounded launch length)
-        const DustArgs A = dust_args(c, dt);
+        DustArgs A = dust_args(c, dt);
+        A.mfin = c->stage;   // 9 * n_grav of stage scratch, unused by this path
         const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + kSmallGrains - 1) / kSmallGrains);
         for (int64_t k = 0; k < nsteps; k += kSmallChunk) {
             const int n = (int)std::min<int64_t>(kSmallChunk, nsteps - k);
@@ -1849,6 +1868,8 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
             case 4: k_small_steps<4><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
             default: k_small_steps<0><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
             }
+            DDYN_HIP(c, hipGetLastError());
+            k_small_commit<<<1, 64, 0, c->stream>>>(A);
             DDYN_HIP(c, hipGetLastError());
         }
         return DDYN_OK;

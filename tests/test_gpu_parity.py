@@ -138,6 +138,30 @@ def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps):
     assert pos_rel_err(out[0], ref).max() <= 1e-10
 
 
+def test_fused_small_steps_more_workgroups_than_fit_at_once(engine_cls):
+    """200,000 test particles make 3,125 workgroups of the multi-step kernel,
+    more than the GPU holds at once, so some start after others have finished:
+    every one must still integrate the massive set from the call's initial
+    state.  Single-step calls (the shortest launches) and a longer call, against
+    one launch per kernel, bit for bit, and the planets against the oracle."""
+    w = synth.config1(n_dust=200_000)
+    calls = [1, 1, 1, 7]
+    out = []
+    for schedule in (0, 1):
+        s = w.state.copy()
+        with engine_cls() as eng:
+            eng.upload(s, replace(w.params, schedule=schedule))
+            for n in calls:
+                eng.step(w.dt, n)
+            eng.download(s)
+        out.append(s)
+    for k in ("x", "y", "z", "vx", "vy", "vz", "N"):
+        assert np.array_equal(getattr(out[0], k), getattr(out[1], k)), k
+    massive = subset(w.state, range(w.state.n_grav), [])
+    ref = O.step(w.params, massive, w.dt, sum(calls))
+    assert pos_rel_err(out[0], ref, slice(0, w.state.n_grav)).max() <= 1e-10
+
+
 @pytest.mark.parametrize("precision", [FP64, FP32_FORCE])
 def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision):
     """Swarm + grains: the grain kernel of step k beside the massive stages of
