@@ -1775,7 +1775,9 @@ int ddyn_upload_state(ddyn_ctx* c, int64_t n_grav, int64_t n_dust, const double*
     if (c->have_params && c->prm.n_planets > n_grav)
         return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
     DDYN_HIP(c, hipSetDevice(c->device));
+    // nothing queued on either of the context's streams may still use the buffers
     DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    DDYN_HIP(c, hipStreamSynchronize(c->gstream));
     free_state(c);
     const size_t na = (size_t)(n_grav + n_dust), ng = (size_t)n_grav;
     double** arr[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->mu, &c->N};

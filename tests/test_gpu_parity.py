@@ -166,12 +166,15 @@ def test_fused_small_steps_more_workgroups_than_fit_at_once(engine_cls):
 def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision):
     """Swarm + grains: the grain kernel of step k beside the massive stages of
     step k+1 (double-buffered stage positions) equals one launch per kernel in
-    order, bit for bit, over two calls with grain removals, and the forces at
-    the end state agree."""
+    order, bit for bit, over two calls with grain removals at the first step
+    and at the fifth, and the forces at the end state agree."""
     w = synth.config3(n_dust=3000, n_swarm=700, precision=precision)
     st = w.state.copy()
     g0 = st.n_grav
-    st.x[g0], st.y[g0], st.z[g0] = st.x[0] + 0.05, st.y[0], st.z[0]   # sublimates (Rmin)
+    st.x[g0], st.y[g0], st.z[g0] = st.x[0] + 0.05, st.y[0], st.z[0]   # sublimates (Rmin) at once
+    # at rest 1 AU from the star: falls inside Rmin after about five steps
+    st.x[g0 + 1], st.y[g0 + 1], st.z[g0 + 1] = st.x[0] + 1.0, st.y[0], st.z[0]
+    st.vx[g0 + 1], st.vy[g0 + 1], st.vz[g0 + 1] = st.vx[0], st.vy[0], st.vz[0]
     prm = replace(w.params, Rmin=0.1)
     out = []
     for schedule in (0, 1):
@@ -191,6 +194,7 @@ def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision
     assert np.array_equal(out[0][2], out[1][2])
     assert np.array_equal(out[0][3], out[1][3])
     assert out[0][1].N[g0] == 0.0
+    assert out[0][0].N[g0 + 1] == 0.0 and out[0][0].x[g0 + 1] != st.x[g0 + 1]
 
 
 def test_jacobi_integral_gpu(engine_cls):
