@@ -309,6 +309,8 @@ __global__ __launch_bounds__(kWG) void k_stage(StageArgs A)
         az = __builtin_fma(dz, q, az);
     }
     const size_t n = (size_t)A.n_grav;
+    // slice partials in slice order; unrolled so that many loads are in flight
+#pragma unroll 8
     for (int k = 0; k < A.nslice; ++k) {
         const double* pk = A.part + (size_t)k * 3 * n;
         ax += pk[i]; ay += pk[n + i]; az += pk[2 * n + i];
@@ -1266,6 +1268,8 @@ __global__ void k_adj_reduce(AdjArgs A)
     if (item >= 4 * n) return;
     const int st = item / n, j = item % n;
     double s[4] = {0, 0, 0, 0};
+    // wave order; unrolled so that many loads are in flight
+#pragma unroll 8
     for (int w = 0; w < A.nwaves; ++w) {
         const double* p = A.wpart + (((size_t)w * 4 + st) * n + j) * 4;
         for (int c = 0; c < 4; ++c) s[c] += p[c];
@@ -1275,6 +1279,7 @@ __global__ void k_adj_reduce(AdjArgs A)
         double mt = s[3];
         for (int s2 = 1; s2 < 4; ++s2) {
             double acc = 0.0;
+#pragma unroll 8
             for (int w = 0; w < A.nwaves; ++w) acc += A.wpart[(((size_t)w * 4 + s2) * n + j) * 4 + 3];
             mt += acc;
         }
@@ -1293,6 +1298,7 @@ __global__ void k_adj_mstage(AdjArgs A, int with_grains)
     const double h = A.dt;
     const double hh[3] = {h / 2.0, h / 2.0, h};
     double Rb[3] = {0, 0, 0}, mb = 0.0;
+#pragma unroll 8
     for (int y = 0; y < A.nslice; ++y) {
         const double* p = A.part + (size_t)y * 4 * n;
         Rb[0] += p[i]; Rb[1] += p[n + i]; Rb[2] += p[2 * n + i]; mb += p[3 * n + i];
