@@ -1261,29 +1261,34 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
 }
 
 // Sum the per-wave grain partials in wave order: Rbg[st][c][j], mug[j].
-__global__ void k_adj_reduce(AdjArgs A)
+// One wave per (stage, massive body): lane l sums the per-wave pulls of grain
+// waves l, l + 64, ... in order, and a fixed shuffle tree adds the 64 lane
+// sums, so the result is deterministic.  (The number of items, 4 n_grav, is
+// small next to the number of grain waves, so a thread per item would wait
+// through one load latency per grain wave.)
+__global__ __launch_bounds__(kWG) void k_adj_reduce(AdjArgs A)
 {
-    const int item = blockIdx.x * kWG + threadIdx.x;   // (st, j)
+    const int item = blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);   // (st, j)
+    const int lane = threadIdx.x & 63;
     const int n = A.n_grav;
-    if (item >= 4 * n) return;
+    if (item >= 4 * n) return;   // whole waves
     const int st = item / n, j = item % n;
     double s[4] = {0, 0, 0, 0};
-    // wave order; unrolled so that many loads are in flight
-#pragma unroll 8
-    for (int w = 0; w < A.nwaves; ++w) {
+    for (int w = lane; w < A.nwaves; w += 64) {
         const double* p = A.wpart + (((size_t)w * 4 + st) * n + j) * 4;
         for (int c = 0; c < 4; ++c) s[c] += p[c];
     }
-    for (int c = 0; c < 3; ++c) A.Rbg[((size_t)st * 3 + c) * n + j] = s[c];
+    for (int c = 0; c < 4; ++c) s[c] = wave_sum(s[c]);
+    if (lane == 0)
+        for (int c = 0; c < 3; ++c) A.Rbg[((size_t)st * 3 + c) * n + j] = s[c];
     if (st == 0) {
         double mt = s[3];
         for (int s2 = 1; s2 < 4; ++s2) {
             double acc = 0.0;
-#pragma unroll 8
-            for (int w = 0; w < A.nwaves; ++w) acc += A.wpart[(((size_t)w * 4 + s2) * n + j) * 4 + 3];
-            mt += acc;
+            for (int w = lane; w < A.nwaves; w += 64) acc += A.wpart[(((size_t)w * 4 + s2) * n + j) * 4 + 3];
+            mt += wave_sum(acc);
         }
-        A.mug[j] = mt;
+        if (lane == 0) A.mug[j] = mt;
     }
 }
 
@@ -2041,7 +2046,7 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
             k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
             if (nd) {
                 k_adj_dust<<<(unsigned)((nd + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
-                k_adj_reduce<<<(unsigned)((4 * ng + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
+                k_adj_reduce<<<(unsigned)((4 * ng + kWG / 64 - 1) / (kWG / 64)), kWG, 0, c->stream>>>(A);
             }
             const int wg = nd ? 1 : 0;
             for (int st = 3; st >= 0; --st) {
