@@ -133,6 +133,24 @@ def state_update(w, kt, steps):
             "frac_hbm_peak": gbs / PEAK_HBM_GBS}
 
 
+def jacobi_drift(n_dust=4096, nsteps=1000):
+    """Jacobi-integral drift of the planetary system's circular restricted
+    variant (one planet on a circular orbit, which the two eccentric planets of
+    the benchmark system do not allow): max and median |dC/C0| over the test
+    particles after nsteps plain FP64 steps on the GPU."""
+    w = synth.config1_jacobi(n_dust=n_dust, nsteps=nsteps)
+    n = w.meta["mean_motion"]
+    st = w.state.copy()
+    c0 = synth.jacobi_constants(st, n)
+    with Engine(0) as eng:
+        eng.upload(st, w.params)
+        eng.step(w.dt, w.nsteps)
+        eng.download(st)
+    d = np.abs((synth.jacobi_constants(st, n) - c0) / c0)
+    return {"max": float(d.max()), "median": float(np.median(d)), "steps": nsteps, "test_particles": n_dust,
+            "note": "circular restricted variant: 1 Msun star, 9.5e-4 Msun planet on a circular orbit at 5.2 AU"}
+
+
 def force_error(w, a_gpu, nsample=2048, seed=7):
     """Median/max relative acceleration error of the GPU against the FP64
     oracle on sampled massive targets (grains: all of a sampled subset)."""
@@ -382,6 +400,8 @@ def main():
                           "note": "one upload and one download of the whole state around the timed steps; "
                                   "value excludes them (state resident in HBM)"},
     }
+    if args.workload == "cfg1":
+        out["jacobi_drift"] = jacobi_drift()
     if not args.no_cpu:
         cb = cpu_baseline(w32)
         out["cpu_baseline"] = cb

@@ -115,6 +115,17 @@ def config1_jacobi(n_dust=256, seed=1011, nsteps=1000, dt=0.01, mp=9.5e-4, ap=5.
                     {"mean_motion": n, "desc": "circular restricted three-body test"})
 
 
+def jacobi_constants(st, mean_motion):
+    """Jacobi integral E - n Lz of every test particle of a config1_jacobi
+    state (star at row 0, planet at row 1, barycentre at rest at the origin)."""
+    g = slice(2, None)
+    v2 = st.vx[g] ** 2 + st.vy[g] ** 2 + st.vz[g] ** 2
+    r0 = np.sqrt((st.x[g] - st.x[0]) ** 2 + (st.y[g] - st.y[0]) ** 2 + (st.z[g] - st.z[0]) ** 2)
+    r1 = np.sqrt((st.x[g] - st.x[1]) ** 2 + (st.y[g] - st.y[1]) ** 2 + (st.z[g] - st.z[1]) ** 2)
+    energy = 0.5 * v2 - st.mu[0] / r0 - st.mu[1] / r1
+    return energy - mean_motion * (st.x[g] * st.vy[g] - st.y[g] * st.vx[g])
+
+
 def config2(n_swarm=65536, seed=1002, nsteps=200, dt=0.02, precision=FP32_FORCE, eps=1e-3):
     """Self-gravitating swarm: 1 Msun star and n_swarm bodies with masses
     log-uniform in [1e-10, 1e-7] Msun, a~U(20,40), e~Rayleigh(0.02),

@@ -202,13 +202,7 @@ def test_jacobi_integral_gpu(engine_cls):
     every test particle is conserved to 1e-10 over 1,000 steps on the GPU."""
     w = synth.config1_jacobi(n_dust=512, nsteps=1000)
     n = w.meta["mean_motion"]
-
-    def jacobi(s):
-        g = slice(2, None)
-        v2 = s.vx[g] ** 2 + s.vy[g] ** 2 + s.vz[g] ** 2
-        r0 = np.sqrt((s.x[g] - s.x[0]) ** 2 + (s.y[g] - s.y[0]) ** 2 + (s.z[g] - s.z[0]) ** 2)
-        r1 = np.sqrt((s.x[g] - s.x[1]) ** 2 + (s.y[g] - s.y[1]) ** 2 + (s.z[g] - s.z[1]) ** 2)
-        return 0.5 * v2 - s.mu[0] / r0 - s.mu[1] / r1 - n * (s.x[g] * s.vy[g] - s.y[g] * s.vx[g])
+    jacobi = lambda s: synth.jacobi_constants(s, n)  # noqa: E731
 
     st = w.state.copy()
     c0 = jacobi(st)
