@@ -213,7 +213,21 @@ def cpu_baseline(w, target_s=12.0):
         sample = f"{steps} full FP64 RK4 steps"
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
-            "sample": sample + f" in {dt:.1f} s"}
+            "sample": sample + f" in {dt:.1f} s", "cpu": host_cpu()}
+
+
+def host_cpu():
+    """CPU model and logical CPU count of this host (Linux), for the baseline."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
 
 
 def adjoint_run(name, precision=FP32_FORCE):
