@@ -150,7 +150,8 @@ int ddyn_synchronize(ddyn_ctx *ctx);
 /* Kernel timing with HIP events around each launch (on the stream it runs on).  Classes: */
 #define DDYN_KCLASS_PAIRS 0   /* massive swarm-pair kernel */
 #define DDYN_KCLASS_STAGE 1   /* massive stage epilogue (direct terms + RK4 update) */
-#define DDYN_KCLASS_DUST 2    /* fused four-stage grain kernel */
+#define DDYN_KCLASS_DUST 2    /* fused four-stage grain kernel; also the star-and-planets
+                                 multi-step kernel (whole steps per launch) */
 #define DDYN_KCLASS_COUNT 3
 int ddyn_set_timing(ddyn_ctx *ctx, int enable);
 /* Totals since timing was enabled (synchronises). */
