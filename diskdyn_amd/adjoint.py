@@ -22,12 +22,17 @@ from .units import GRAV
 
 class Runner:
     """Fixed parts of a run: the template state (ids, N, beta_sw, grain masses),
-    parameters, step size, step count and checkpoint interval."""
+    parameters, step size, step count and checkpoint interval.
 
-    def __init__(self, state, params, dt, nsteps, ckpt_every=0, device=0, report=None):
+    With tape=True a forward pass that will be differentiated records the
+    adjoint's tape on the device (ddyn_adjoint_record) and keeps its context;
+    the reverse pass then needs no second forward pass."""
+
+    def __init__(self, state, params, dt, nsteps, ckpt_every=0, device=0, report=None, tape=False):
         self.state, self.params = state, params
         self.dt, self.nsteps, self.ckpt_every, self.device = dt, nsteps, ckpt_every, device
         self.report = report
+        self.tape = tape
 
     def make_state(self, r0, v0, mu, beta):
         st = self.state.copy()
@@ -51,10 +56,24 @@ class Runner:
     def forward(self, st):
         return self.forward_with_checkpoints(st)[0]
 
-    def forward_with_checkpoints(self, st):
-        """Final state, and in windowed mode the host states at the window
-        starts (the adjoint's checkpoints, so that it need not run the forward
-        pass again); None otherwise."""
+    def forward_with_checkpoints(self, st, record=False):
+        """Final state, and what the reverse pass can reuse: with tape and
+        record, the Engine holding the recorded tape (the caller hands it to
+        adjoint(), which closes it); in windowed mode the host states at the
+        window starts; None otherwise."""
+        if self.tape and record:
+            t0 = time.perf_counter()
+            eng = Engine(self.device)
+            try:
+                eng.upload(st, self.params)
+                eng.adjoint_record(self.dt, self.nsteps, self.ckpt_every)
+                out = eng.download(st.copy())
+            except BaseException:
+                eng.close()
+                raise
+            if self.report is not None:
+                self.report(f"forward {self.nsteps} steps, adjoint tape recorded ({time.perf_counter() - t0:.1f} s)")
+            return out, eng
         ckpts = None
         with Engine(self.device) as eng:
             eng.upload(st, self.params)
@@ -80,7 +99,18 @@ class Runner:
         (ddyn_adjoint on each checkpoint window, the adjoint state passed from
         one window to the previous) and `report` is called after each window.
         `ckpts` are the window-start states from forward_with_checkpoints;
-        without them the forward pass runs again to make them."""
+        without them the forward pass runs again to make them.  If `ckpts` is
+        the Engine of a recorded forward pass (tape mode), the gradients come
+        from its tape and the Engine is closed."""
+        if isinstance(ckpts, Engine):
+            t0 = time.perf_counter()
+            try:
+                g = ckpts.adjoint_reverse(lam_r, lam_v)
+            finally:
+                ckpts.close()
+            if self.report is not None:
+                self.report(f"adjoint of {self.nsteps} steps from the tape ({time.perf_counter() - t0:.1f} s)")
+            return g
         if not self.windowed:
             with Engine(self.device) as eng:
                 eng.upload(st, self.params)
@@ -112,7 +142,7 @@ class Trajectory(torch.autograd.Function):
     @staticmethod
     def forward(ctx, r0, v0, mu, beta, runner):
         st = runner.make_state(r0, v0, mu, beta)
-        out, ckpts = runner.forward_with_checkpoints(st)
+        out, ckpts = runner.forward_with_checkpoints(st, record=any(ctx.needs_input_grad[:4]))
         ctx.runner = runner
         ctx.state = st
         ctx.ckpts = ckpts

@@ -1402,6 +1402,15 @@ struct ddyn_ctx {
     double* part = nullptr;      // nslice * 3 * n_grav
     int nslice = 0, slice_len = 0, tpt = 1;
     double* acc = nullptr;       // 3 * n_all accel scratch
+    // adjoint tape (ddyn_adjoint_record): a checkpoint at the start of every
+    // segment (7 * n_all each), the last segment's per-step states (7 * n_all)
+    // and stage-1..3 massive positions (3 * n_grav), and the final state
+    double *tape_ck = nullptr, *tape_seg = nullptr, *tape_fin = nullptr;
+    double4* tape_stg = nullptr;
+    double tape_dt = 0.0;
+    int64_t tape_n = 0, tape_ck_every = 0;
+    bool have_tape = false;
+    bool tape_last = false;      // tape_seg / tape_stg still hold the last segment
     int32_t* rem = nullptr;      // n_dust accel scratch
     // grain coefficients
     double *beta_p = nullptr, *betasw_p = nullptr;
@@ -1444,11 +1453,26 @@ void dfree(T*& p)
     p = nullptr;
 }
 
+// Frees the adjoint tape, after the work queued on the main stream that may
+// still write it; every call that changes the state or the physics makes the
+// tape stale.
+void drop_tape(ddyn_ctx* c)
+{
+    if (c->tape_ck || c->tape_seg || c->tape_fin || c->tape_stg) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    dfree(c->tape_ck); dfree(c->tape_seg); dfree(c->tape_fin); dfree(c->tape_stg);
+    c->have_tape = false;
+}
+
 void free_state(ddyn_ctx* c)
 {
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->vx); dfree(c->vy); dfree(c->vz);
     dfree(c->mu); dfree(c->N); dfree(c->id);
     for (int s = 0; s < 4; ++s) { dfree(c->p64[s]); dfree(c->p32[s]); dfree(c->q64[s]); dfree(c->q32[s]); }
+    dfree(c->tape_ck); dfree(c->tape_seg); dfree(c->tape_fin); dfree(c->tape_stg);
+    c->have_tape = false;
     dfree(c->stage); dfree(c->part); dfree(c->acc); dfree(c->rem);
     dfree(c->beta_p); dfree(c->betasw_p);
     c->use_beta_p = false; c->have_betasw_p = false;
@@ -1759,6 +1783,7 @@ int ddyn_set_params(ddyn_ctx* c, const ddyn_params* p)
     if (!(p->eps >= 0.0) || !(p->Rmin >= 0.0)) return fail(c, DDYN_ERR_ARG, "eps and Rmin must be >= 0");
     if (p->SWQ == 1 && !(p->vsw > 0.0)) return fail(c, DDYN_ERR_ARG, "SWQ needs vsw > 0");
     if (c->x && p->n_planets > c->n_grav) return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
+    drop_tape(c);
     c->prm = *p;
     c->have_params = true;
     c->fuse_small = p->schedule == 0;
@@ -1834,6 +1859,7 @@ int ddyn_upload_bin_tables(ddyn_ctx* c, int32_t n_bins, const float* beta, const
     if (n_bins > 0 && (!beta || !betasw || !q)) return fail(c, DDYN_ERR_ARG, "tables must not be NULL");
     DDYN_HIP(c, hipSetDevice(c->device));
     DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    drop_tape(c);
     if (n_bins > 0) {
         DDYN_HIP(c, hipMemcpy(c->beta_t, beta, n_bins * sizeof(float), hipMemcpyHostToDevice));
         DDYN_HIP(c, hipMemcpy(c->betasw_t, betasw, n_bins * sizeof(float), hipMemcpyHostToDevice));
@@ -1849,6 +1875,7 @@ int ddyn_upload_grain_beta(ddyn_ctx* c, const double* beta, const double* betasw
     if (!c->x) return fail(c, DDYN_ERR_STATE, "ddyn_upload_state has not been called");
     DDYN_HIP(c, hipSetDevice(c->device));
     DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    drop_tape(c);
     if (!beta) { c->use_beta_p = false; c->have_betasw_p = false; return DDYN_OK; }
     const size_t nd = (size_t)c->n_dust;
     if (nd) {
@@ -1866,6 +1893,7 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
     if (rc) return rc;
     if (!(dt > 0.0) || nsteps < 0) return fail(c, DDYN_ERR_ARG, "need dt > 0 and nsteps >= 0");
     DDYN_HIP(c, hipSetDevice(c->device));
+    drop_tape(c);
     if (n_swarm(c) == 0 && c->n_grav <= kSmallMax && c->fuse_small) {
         // star + planets only: many steps per launch (bounded launch length)
         DustArgs A = dust_args(c, dt);
@@ -1952,51 +1980,115 @@ using DevBuf4 = DevBufT<double4>;
 
 }  // namespace
 
-int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, const double* lam_r,
-                 const double* lam_v, double* g_r, double* g_v, double* g_mu, double* g_beta)
+namespace {
+
+// Segment length and count of an adjoint run (ckpt_every <= 0 or >= nsteps:
+// one segment).
+int64_t seg_len(int64_t nsteps, int64_t ckpt_every)
 {
-    int rc = check_ready(c);
-    if (rc) return rc;
-    if (!lam_r || !lam_v || !g_r || !g_v || !g_mu) return fail(c, DDYN_ERR_ARG, "adjoint arrays must not be NULL");
-    if (!(dt > 0.0) || nsteps < 0) return fail(c, DDYN_ERR_ARG, "need dt > 0 and nsteps >= 0");
-    if (c->prm.FLQ == 1) return fail(c, DDYN_ERR_ARG, "the adjoint does not support the Lorentz force (FLQ)");
-    DDYN_HIP(c, hipSetDevice(c->device));
+    return (ckpt_every > 0 && ckpt_every < nsteps) ? ckpt_every : (nsteps > 0 ? nsteps : 1);
+}
+int64_t seg_count(int64_t nsteps, int64_t ck) { return nsteps > 0 ? (nsteps + ck - 1) / ck : 0; }
+
+void save_state(ddyn_ctx* c, double* dst)
+{
+    const size_t na = (size_t)(c->n_grav + c->n_dust);
+    k_copy_state<<<(unsigned)((na + kWG - 1) / kWG), kWG, 0, c->stream>>>(na, c->x, c->y, c->z, c->vx, c->vy,
+                                                                          c->vz, c->N, dst);
+}
+
+void restore_state(ddyn_ctx* c, const double* src)
+{
+    const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav;
+    k_restore_state<<<(unsigned)((na + kWG - 1) / kWG), kWG, 0, c->stream>>>(na, src, c->x, c->y, c->z, c->vx,
+                                                                             c->vy, c->vz, c->N);
+    k_init_pos<<<(unsigned)((ng + kWG - 1) / kWG), kWG, 0, c->stream>>>((int)ng, c->x, c->y, c->z, c->mu,
+                                                                        c->p64[0], c->p32[0]);
+}
+
+// Steps k0..k1-1 of a segment from the current state, keeping each step's state
+// in tape_seg and its massive positions of stages 1-3 in tape_stg (the stage
+// kernels write them there directly).  The last step is completed only when
+// `finish` is set: the reverse sweep needs the states at the steps' starts, and
+// the step after a segment is the next segment's checkpoint.
+int tape_segment(ddyn_ctx* c, double dt, int64_t k0, int64_t k1, bool finish)
+{
+    const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav;
+    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
+    double4* const own64[3] = {c->p64[1], c->p64[2], c->p64[3]};
+    int rc = DDYN_OK;
+    for (int64_t k = k0; k < k1 && !rc; ++k) {
+        save_state(c, c->tape_seg + (size_t)(k - k0) * 7 * na);
+        for (int st = 1; st <= 3; ++st) c->p64[st] = c->tape_stg + ((size_t)(k - k0) * 3 + (size_t)(st - 1)) * ng;
+        for (int st = 0; st < 3 && !rc; ++st) {   // massive stages 0-2 of step k
+            if (!(rc = launch_pairs(c, st))) rc = launch_stage(c, st, stage_args(c, hs[st]));
+        }
+        if (!rc && (k + 1 < k1 || finish)) {      // the rest of the step
+            if (!(rc = launch_pairs(c, 3)) && !(rc = launch_dust(c, dust_args(c, dt), false)))
+                rc = launch_stage(c, 3, stage_args(c, dt / 6.0));
+        }
+        for (int st = 1; st <= 3; ++st) c->p64[st] = own64[st - 1];
+    }
+    return rc;
+}
+
+// Forward pass of the adjoint: a checkpoint at the start of every segment, the
+// last segment taped (tape_segment), the final state kept.  The context ends at
+// the state after nsteps.
+int record_tape(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every)
+{
+    drop_tape(c);
+    const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav;
+    const int64_t ck = seg_len(nsteps, ckpt_every), nseg = seg_count(nsteps, ck);
+    if (dalloc(c, c->tape_ck, (size_t)std::max<int64_t>(nseg, 1) * 7 * na) ||
+        dalloc(c, c->tape_seg, (size_t)ck * 7 * na) || dalloc(c, c->tape_stg, (size_t)ck * 3 * ng) ||
+        dalloc(c, c->tape_fin, 7 * na)) {
+        drop_tape(c);
+        return DDYN_ERR_NOMEM;
+    }
+    int rc;
+    const int64_t klast = nseg > 0 ? (nseg - 1) * ck : 0;
+    for (int64_t k = 0; k < klast; ++k) {
+        if (k % ck == 0) save_state(c, c->tape_ck + (size_t)(k / ck) * 7 * na);
+        if ((rc = one_step(c, dt))) { drop_tape(c); return rc; }
+    }
+    if (nseg > 0) {
+        save_state(c, c->tape_ck + (size_t)(nseg - 1) * 7 * na);
+        if ((rc = tape_segment(c, dt, klast, nsteps, true))) { drop_tape(c); return rc; }
+    }
+    save_state(c, c->tape_fin);
+    DDYN_HIP(c, hipGetLastError());
+    c->tape_dt = dt;
+    c->tape_n = nsteps;
+    c->tape_ck_every = ck;
+    c->have_tape = true;
+    c->tape_last = nseg > 0;
+    return DDYN_OK;
+}
+
+// Reverse sweep over a recorded tape: the last segment from the tape, every
+// earlier one recomputed from its checkpoint into the same buffers (after which
+// the last segment is no longer on the tape, and a later sweep recomputes it
+// too).  Ends at the recorded final state, with the tape still valid.
+int reverse_tape(ddyn_ctx* c, const double* lam_r, const double* lam_v, double* g_r, double* g_v,
+                 double* g_mu, double* g_beta)
+{
+    const double dt = c->tape_dt;
+    const int64_t nsteps = c->tape_n, ck = c->tape_ck_every, nseg = seg_count(nsteps, ck);
     const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav, nd = (size_t)c->n_dust;
-    const int64_t ck = (ckpt_every > 0 && ckpt_every < nsteps) ? ckpt_every : (nsteps > 0 ? nsteps : 1);
-    const int64_t nseg = nsteps > 0 ? (nsteps + ck - 1) / ck : 0;
     const int nwaves = (int)((nd + kWG - 1) / kWG) * (kWG / 64);
     // VJP source split over [0, n_grav)
     const int tblocks = (int)((ng + kWG - 1) / kWG);
     int vsl = std::max(1, std::min((kTargetWGs + tblocks - 1) / tblocks, (int)(ng / kMinSlice)));
     int vlen = (int)(((ng + vsl - 1) / vsl + kTile - 1) / kTile) * kTile;
     vsl = (int)((ng + vlen - 1) / vlen);
-    DevBuf ckb, segb, finb, rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart;
-    DevBuf4 stgb;   // massive positions of stages 1-3 of every step of the current segment
-    if (dalloc(c, ckb.p, (size_t)std::max<int64_t>(nseg, 1) * 7 * na) || dalloc(c, segb.p, (size_t)ck * 7 * na) ||
-        dalloc(c, stgb.p, (size_t)ck * 3 * ng) ||
-        dalloc(c, finb.p, 7 * na) || dalloc(c, rbb.p, 3 * na) || dalloc(c, vbb.p, 3 * na) ||
+    DevBuf rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart;
+    if (dalloc(c, rbb.p, 3 * na) || dalloc(c, vbb.p, 3 * na) ||
         dalloc(c, Vbm.p, 12 * ng) || dalloc(c, Abm.p, 12 * ng) || dalloc(c, Rbg.p, 12 * ng) ||
         dalloc(c, mug.p, ng) || dalloc(c, gmu.p, ng) || dalloc(c, gbeta.p, std::max<size_t>(nd, 1)) ||
         dalloc(c, wpart.p, (size_t)std::max(nwaves, 1) * 16 * ng) || dalloc(c, vpart.p, (size_t)vsl * 4 * ng))
         return DDYN_ERR_NOMEM;
-    const unsigned bn = (unsigned)((na + kWG - 1) / kWG), bg = (unsigned)((ng + kWG - 1) / kWG);
-    auto save = [&](double* dst) {
-        k_copy_state<<<bn, kWG, 0, c->stream>>>(na, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N, dst);
-    };
-    auto restore = [&](const double* src) {
-        k_restore_state<<<bn, kWG, 0, c->stream>>>(na, src, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->N);
-        k_init_pos<<<bg, kWG, 0, c->stream>>>((int)ng, c->x, c->y, c->z, c->mu, c->p64[0], c->p32[0]);
-    };
-    // forward pass with checkpoints, up to the start of the last segment (that
-    // segment's recomputation below runs its steps to the end)
-    const int64_t klast = nseg > 0 ? (nseg - 1) * ck : 0;
-    for (int64_t k = 0; k < klast; ++k) {
-        if (k % ck == 0) save(ckb.p + (size_t)(k / ck) * 7 * na);
-        if ((rc = one_step(c, dt))) return rc;
-    }
-    if (nseg > 0) save(ckb.p + (size_t)(nseg - 1) * 7 * na);
-    else save(finb.p);
-    DDYN_HIP(c, hipGetLastError());
+    const unsigned bg = (unsigned)((ng + kWG - 1) / kWG);
     DDYN_HIP(c, hipMemcpyAsync(rbb.p, lam_r, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
     DDYN_HIP(c, hipMemcpyAsync(vbb.p, lam_v, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
     DDYN_HIP(c, hipMemsetAsync(gmu.p, 0, ng * sizeof(double), c->stream));
@@ -2015,34 +2107,18 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
     A.light = c->prm.light > 0.0 ? c->prm.light : kLightAU;
     A.rb = rbb.p; A.vb = vbb.p; A.Vbm = Vbm.p; A.Abm = Abm.p; A.Rbg = Rbg.p; A.mug = mug.p;
     A.wpart = wpart.p; A.part = vpart.p; A.g_mu = gmu.p; A.g_beta = gbeta.p;
-    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
 
-    // The segment's forward recomputation keeps, per step, the state and the
-    // massive positions of stages 1-3 (written straight into stgb by pointing
-    // the stage buffers there), so the reverse sweep recomputes nothing.
-    double4* const own64[3] = {c->p64[1], c->p64[2], c->p64[3]};
-    auto stage_slot = [&](int64_t kk, int st) { return stgb.p + ((size_t)kk * 3 + (size_t)(st - 1)) * ng; };
+    int rc;
     for (int64_t s = nseg - 1; s >= 0; --s) {
         const int64_t k0 = s * ck, k1 = std::min(k0 + ck, nsteps);
-        restore(ckb.p + (size_t)s * 7 * na);
-        for (int64_t k = k0; k < k1; ++k) {
-            save(segb.p + (size_t)(k - k0) * 7 * na);
-            for (int st = 1; st <= 3; ++st) c->p64[st] = stage_slot(k - k0, st);
-            for (int st = 0; st < 3; ++st) {           // massive stages 0-2 of step k
-                if ((rc = launch_pairs(c, st))) break;
-                if ((rc = launch_stage(c, st, stage_args(c, hs[st])))) break;
-            }
-            if (!rc && (k + 1 < k1 || s == nseg - 1)) {   // the rest of the step
-                if (!(rc = launch_pairs(c, 3)) && !(rc = launch_dust(c, dust_args(c, dt), false)))
-                    rc = launch_stage(c, 3, stage_args(c, dt / 6.0));
-            }
-            for (int st = 1; st <= 3; ++st) c->p64[st] = own64[st - 1];
-            if (rc) return rc;
+        if (s < nseg - 1 || !c->tape_last) {   // recompute the segment
+            restore_state(c, c->tape_ck + (size_t)s * 7 * na);
+            if ((rc = tape_segment(c, dt, k0, k1, s == nseg - 1))) return rc;
+            c->tape_last = false;
         }
-        if (s == nseg - 1) save(finb.p);              // the forward final state
         for (int64_t k = k1 - 1; k >= k0; --k) {
-            restore(segb.p + (size_t)(k - k0) * 7 * na);
-            for (int st = 1; st <= 3; ++st) A.p64[st] = stage_slot(k - k0, st);
+            restore_state(c, c->tape_seg + (size_t)(k - k0) * 7 * na);
+            for (int st = 1; st <= 3; ++st) A.p64[st] = c->tape_stg + ((size_t)(k - k0) * 3 + (size_t)(st - 1)) * ng;
             k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
             if (nd) {
                 k_adj_dust<<<(unsigned)((nd + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
@@ -2070,7 +2146,7 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
             DDYN_HIP(c, hipGetLastError());
         }
     }
-    restore(finb.p);
+    restore_state(c, c->tape_fin);
     DDYN_HIP(c, hipGetLastError());
     DDYN_HIP(c, hipStreamSynchronize(c->stream));
     DDYN_HIP(c, hipMemcpy(g_r, rbb.p, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
@@ -2078,6 +2154,50 @@ int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, con
     DDYN_HIP(c, hipMemcpy(g_mu, gmu.p, ng * sizeof(double), hipMemcpyDeviceToHost));
     if (g_beta && nd) DDYN_HIP(c, hipMemcpy(g_beta, gbeta.p, nd * sizeof(double), hipMemcpyDeviceToHost));
     return DDYN_OK;
+}
+
+int check_adjoint_args(ddyn_ctx* c, double dt, int64_t nsteps)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!(dt > 0.0) || nsteps < 0) return fail(c, DDYN_ERR_ARG, "need dt > 0 and nsteps >= 0");
+    if (c->prm.FLQ == 1) return fail(c, DDYN_ERR_ARG, "the adjoint does not support the Lorentz force (FLQ)");
+    return DDYN_OK;
+}
+
+}  // namespace
+
+int ddyn_adjoint(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, const double* lam_r,
+                 const double* lam_v, double* g_r, double* g_v, double* g_mu, double* g_beta)
+{
+    int rc = check_adjoint_args(c, dt, nsteps);
+    if (rc) return rc;
+    if (!lam_r || !lam_v || !g_r || !g_v || !g_mu) return fail(c, DDYN_ERR_ARG, "adjoint arrays must not be NULL");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    if ((rc = record_tape(c, dt, nsteps, ckpt_every))) return rc;
+    rc = reverse_tape(c, lam_r, lam_v, g_r, g_v, g_mu, g_beta);
+    drop_tape(c);
+    return rc;
+}
+
+int ddyn_adjoint_record(ddyn_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every)
+{
+    int rc = check_adjoint_args(c, dt, nsteps);
+    if (rc) return rc;
+    DDYN_HIP(c, hipSetDevice(c->device));
+    return record_tape(c, dt, nsteps, ckpt_every);
+}
+
+int ddyn_adjoint_reverse(ddyn_ctx* c, const double* lam_r, const double* lam_v, double* g_r, double* g_v,
+                         double* g_mu, double* g_beta)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!c->have_tape)
+        return fail(c, DDYN_ERR_STATE, "no adjoint tape: call ddyn_adjoint_record after the last change of state");
+    if (!lam_r || !lam_v || !g_r || !g_v || !g_mu) return fail(c, DDYN_ERR_ARG, "adjoint arrays must not be NULL");
+    DDYN_HIP(c, hipSetDevice(c->device));
+    return reverse_tape(c, lam_r, lam_v, g_r, g_v, g_mu, g_beta);
 }
 
 int ddyn_energy(ddyn_ctx* c, double* out7)

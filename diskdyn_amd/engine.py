@@ -46,6 +46,8 @@ SIGNATURES = {
     "ddyn_download_state": (ctypes.c_int, [_VP] + [_D] * 7),
     "ddyn_energy": (ctypes.c_int, [_VP, _D]),
     "ddyn_adjoint": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int64, ctypes.c_int64] + [_D] * 6),
+    "ddyn_adjoint_record": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int64, ctypes.c_int64]),
+    "ddyn_adjoint_reverse": (ctypes.c_int, [_VP] + [_D] * 6),
     "ddyn_synchronize": (ctypes.c_int, [_VP]),
     "ddyn_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ddyn_read_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _I64, _D]),
@@ -199,6 +201,25 @@ class Engine:
         g_beta = np.zeros(max(self.n_dust, 1))
         self._check(self.lib.ddyn_adjoint(self.ctx, float(dt), int(nsteps), int(ckpt_every), _dp(lr), _dp(lv),
                                           _dp(g_r), _dp(g_v), _dp(g_mu), _dp(g_beta)), "ddyn_adjoint")
+        return (g_r.reshape(3, n).T.copy(), g_v.reshape(3, n).T.copy(), g_mu, g_beta[: self.n_dust])
+
+    def adjoint_record(self, dt, nsteps, ckpt_every=0):
+        """Forward pass of the adjoint, keeping its tape on the device; the
+        device state ends at the state after nsteps."""
+        self._check(self.lib.ddyn_adjoint_record(self.ctx, float(dt), int(nsteps), int(ckpt_every)),
+                    "ddyn_adjoint_record")
+
+    def adjoint_reverse(self, lam_r, lam_v):
+        """Gradients of the recorded run for dL/d(final r, v) = lam_r, lam_v,
+        as adjoint() returns them."""
+        n = self.n_grav + self.n_dust
+        lr = np.ascontiguousarray(np.asarray(lam_r, dtype=np.float64).T).reshape(3 * n)
+        lv = np.ascontiguousarray(np.asarray(lam_v, dtype=np.float64).T).reshape(3 * n)
+        g_r, g_v = np.empty(3 * n), np.empty(3 * n)
+        g_mu = np.empty(self.n_grav)
+        g_beta = np.zeros(max(self.n_dust, 1))
+        self._check(self.lib.ddyn_adjoint_reverse(self.ctx, _dp(lr), _dp(lv), _dp(g_r), _dp(g_v), _dp(g_mu),
+                                                  _dp(g_beta)), "ddyn_adjoint_reverse")
         return (g_r.reshape(3, n).T.copy(), g_v.reshape(3, n).T.copy(), g_mu, g_beta[: self.n_dust])
 
     def set_timing(self, enable=True):

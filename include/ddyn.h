@@ -144,6 +144,23 @@ int ddyn_adjoint(ddyn_ctx *ctx, double dt, int64_t nsteps, int64_t ckpt_every,
                  const double *lam_r, const double *lam_v, double *g_r, double *g_v,
                  double *g_mu, double *g_beta);
 
+/* ddyn_adjoint in two calls, for a loss that needs the final state before its
+ * gradient lam_r, lam_v is known.  No reference counterpart.
+ * ddyn_adjoint_record runs the forward pass from the current state (the
+ * context ends at the state after nsteps, as ddyn_step would leave it) and
+ * keeps a tape on the device: a checkpoint every ckpt_every steps (<= 0: one
+ * segment), and every step's state and stage positions of the last segment,
+ * 56 B per particle plus 96 B per massive body per step of it.
+ * ddyn_adjoint_reverse then gives what ddyn_adjoint gives for the recorded run,
+ * bit for bit, without a second forward pass (earlier segments are recomputed
+ * from their checkpoints, the last one is on the tape); it ends at the final
+ * state and keeps the tape, so it may be called again with another lam.
+ * ddyn_step, the uploads and ddyn_set_params drop the tape; reversing without
+ * one returns DDYN_ERR_STATE. */
+int ddyn_adjoint_record(ddyn_ctx *ctx, double dt, int64_t nsteps, int64_t ckpt_every);
+int ddyn_adjoint_reverse(ddyn_ctx *ctx, const double *lam_r, const double *lam_v,
+                         double *g_r, double *g_v, double *g_mu, double *g_beta);
+
 /* Wait for all work of the context (the main stream, which every call joins). */
 int ddyn_synchronize(ddyn_ctx *ctx);
 

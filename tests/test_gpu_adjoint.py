@@ -9,9 +9,9 @@ import numpy as np
 import pytest
 import torch
 
-from diskdyn_amd import synth
+from diskdyn_amd import engine, synth
 from diskdyn_amd.adjoint import Runner, Trajectory, planet_initial_state, ring_loss
-from diskdyn_amd.system import FP64
+from diskdyn_amd.system import FP32_FORCE, FP64
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +87,63 @@ def test_segment_chained_adjoint_equals_single_call():
     c = run.adjoint(st.copy(), lam_r, lam_v, ckpts)
     for x, y in zip(a, c):
         assert _rel(np.asarray(y), np.asarray(x)) <= 1e-12
+
+
+@pytest.mark.parametrize("precision,ck", [(FP64, 4), (FP64, 0), (FP32_FORCE, 4)])
+def test_recorded_tape_reverse_equals_single_call(engine_cls, precision, ck):
+    """ddyn_adjoint_record + ddyn_adjoint_reverse give what ddyn_adjoint gives,
+    bit for bit: in one segment (ck = 0) and in three (the last on the tape,
+    the others recomputed); a second reverse with another lam (the last
+    segment then recomputed) too.  The recorded run ends where ddyn_step ends,
+    and a step drops the tape."""
+    w = synth.config3(n_dust=300, n_swarm=40, precision=precision, eps=2e-2)
+    st = w.state
+    nsteps = 9
+    rng = np.random.default_rng(31)
+    lams = [(rng.normal(size=(st.n_all, 3)), rng.normal(size=(st.n_all, 3))) for _ in range(2)]
+    with engine_cls() as eng:
+        eng.upload(st.copy(), w.params)
+        eng.adjoint_record(w.dt, nsteps, ckpt_every=ck)
+        end_rec = eng.download(st.copy())
+        taped = [eng.adjoint_reverse(*lam) for lam in lams]
+        after = eng.download(st.copy())
+        eng.step(w.dt, 1)
+        with pytest.raises(engine.DDynError, match="tape"):
+            eng.adjoint_reverse(*lams[0])
+    for lam, t in zip(lams, taped):
+        with engine_cls() as eng:
+            eng.upload(st.copy(), w.params)
+            ref = eng.adjoint(w.dt, nsteps, *lam, ckpt_every=ck)
+        for x, y in zip(ref, t):
+            assert np.array_equal(np.asarray(x), np.asarray(y))
+    with engine_cls() as eng:
+        eng.upload(st.copy(), w.params)
+        eng.step(w.dt, nsteps)
+        end_step = eng.download(st.copy())
+    for k in ("x", "y", "z", "vx", "vy", "vz", "N"):
+        assert np.array_equal(getattr(end_rec, k), getattr(end_step, k)), k
+        assert np.array_equal(getattr(after, k), getattr(end_step, k)), k
+
+
+def test_trajectory_tape_mode_equals_windowed():
+    """The torch wrapper with the tape (one forward pass) gives the gradients of
+    the windowed Runner (forward, then every window recomputed)."""
+    w = synth.config3(n_dust=200, n_swarm=24, precision=FP64, eps=2e-2)
+    st = w.state
+    beta = torch.from_numpy(st.beta_p.copy())
+    rng = np.random.default_rng(41)
+    wr = torch.from_numpy(rng.normal(size=(st.n_all, 3)))
+    grads = []
+    for kw in ({"tape": True}, {"report": lambda m: None}):
+        run = Runner(st, w.params, w.dt, 8, ckpt_every=3, **kw)
+        r0 = torch.from_numpy(st.pos()).requires_grad_(True)
+        v0 = torch.from_numpy(st.vel()).requires_grad_(True)
+        mu = torch.from_numpy(st.mu[: st.n_grav].copy()).requires_grad_(True)
+        rT, vT = Trajectory.apply(r0, v0, mu, beta, run)
+        ((rT * wr).sum() + (vT * wr).sum()).backward()
+        grads.append((r0.grad.numpy(), v0.grad.numpy(), mu.grad.numpy()))
+    for x, y in zip(*grads):
+        assert _rel(x, y) <= 1e-12
 
 
 def test_gpu_adjoint_directional_fd(engine_cls):
