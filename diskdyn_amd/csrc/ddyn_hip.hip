@@ -1260,7 +1260,7 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
     }
 }
 
-// Sum the per-wave grain partials in wave order: Rbg[st][c][j], mug[j].
+// Sums of the per-wave grain partials: Rbg[st][c][j], mug[j].
 // One wave per (stage, massive body): lane l sums the per-wave pulls of grain
 // waves l, l + 64, ... in order, and a fixed shuffle tree adds the 64 lane
 // sums, so the result is deterministic.  (The number of items, 4 n_grav, is
