@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos
     for (int j0 = jbeg; j0 < jend; j0 += kTile) {
         const int tl = min(kTile, jend - j0);
         __syncthreads();
-        tile[threadIdx.x] = (threadIdx.x < tl) ? pos[j0 + threadIdx.x]
+        tile[threadIdx.x] = ((int)threadIdx.x < tl) ? pos[j0 + threadIdx.x]
                                                : make_double4(kFar, kFar, kFar, 0.0);
         __syncthreads();
 #pragma unroll 2
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos3
     for (int j0 = jbeg; j0 < jend; j0 += kTile) {
         const int tl = min(kTile, jend - j0);
         __syncthreads();
-        tile[threadIdx.x] = (threadIdx.x < tl)
+        tile[threadIdx.x] = ((int)threadIdx.x < tl)
                                 ? pos32[j0 + threadIdx.x]
                                 : make_float4((float)kFar, (float)kFar, (float)kFar, 0.f);
         __syncthreads();
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ p
     for (int j0 = jbeg; j0 < jend; j0 += kTile) {
         const int tl = min(kTile, jend - j0);
         __syncthreads();
-        if (threadIdx.x < tl) {
+        if ((int)threadIdx.x < tl) {
             const int j = j0 + threadIdx.x;
             tp[threadIdx.x] = pos[j];
             ta[threadIdx.x] = make_double4(Ab[j], Ab[n + j], Ab[2 * n + j], 0.0);
@@ -1048,7 +1048,7 @@ __global__ __launch_bounds__(kWG) void k_vjp_pairs32(const double4* __restrict__
     for (int j0 = jbeg; j0 < jend; j0 += kTile) {
         const int tl = min(kTile, jend - j0);
         __syncthreads();
-        if (threadIdx.x < tl) {
+        if ((int)threadIdx.x < tl) {
             const int j = j0 + threadIdx.x;
             const double4 pj = pos[j];
             tp[threadIdx.x] = make_float4((float)pj.x, (float)pj.y, (float)pj.z, (float)pj.w);
@@ -1361,7 +1361,7 @@ __global__ __launch_bounds__(kWG) void k_potential(const double4* __restrict__ p
     for (int j0 = 0; j0 < n_grav; j0 += kTile) {
         const int tl = min(kTile, n_grav - j0);
         __syncthreads();
-        if (threadIdx.x < tl) tile[threadIdx.x] = pos[j0 + threadIdx.x];
+        if ((int)threadIdx.x < tl) tile[threadIdx.x] = pos[j0 + threadIdx.x];
         __syncthreads();
         for (int jj = 0; jj < tl; ++jj) {
             const int j = j0 + jj;
