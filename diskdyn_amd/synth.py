@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from .orbits import coe2rv, kepler_true_anomaly, recalc_com
-from .system import FP32_FORCE, FP64, DiskState, Params, Workload, make_state
+from .system import FP32_FORCE, FP64, Params, Workload, make_state
 from .units import GRAV, LIGHT_AU, kms_to_au_per_yr
 
 DEG = math.pi / 180.0

@@ -3,8 +3,6 @@
 Dynamics run in AU, yr and solar masses; every mass is stored as mu = G*M
 with G = 4*pi^2 (src/ddyn.cuh:30, src/read_in.cu:76).
 """
-import math
-
 MEARTH = 5.97e24            # kg
 AU = 1.49598e11             # m
 MSUN = 1.98892e30           # kg

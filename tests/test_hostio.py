@@ -9,7 +9,6 @@ import numpy as np
 import pytest
 
 from diskdyn_amd import hostio, orbits, synth
-from diskdyn_amd.system import make_state
 from diskdyn_amd.units import AU, GRAV, MSUN, RSUN, YEAR
 
 PARAM = """# test system
