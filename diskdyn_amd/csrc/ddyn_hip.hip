@@ -703,9 +703,10 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// NP > 0 fixes the number of star + planets at compile time, so the pair loops
-// unroll and their independent square-root and division chains overlap (the
-// sums keep their order); NP = 0 takes it from A.
+// NP > 0 fixes the number of star + planets at compile time (1 to 9, the star
+// and up to eight planets), so the pair loops unroll and their independent
+// square-root and division chains overlap (the sums keep their order, so every
+// NP gives the bits of NP = 0); NP = 0 takes it from A.
 template <int NP>
 __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps)
 {
@@ -1907,6 +1908,11 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
             case 2: k_small_steps<2><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
             case 3: k_small_steps<3><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
             case 4: k_small_steps<4><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 5: k_small_steps<5><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 6: k_small_steps<6><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 7: k_small_steps<7><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 8: k_small_steps<8><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            case 9: k_small_steps<9><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
             default: k_small_steps<0><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
             }
             DDYN_HIP(c, hipGetLastError());

@@ -106,14 +106,16 @@ FUSED = {
                                  params=replace(SMALL["cfg1"]().params, n_planets=1)),
     "bodies4": lambda: _planets_only(4),
     "bodies6": lambda: _planets_only(6),
+    "bodies9": lambda: _planets_only(9),
+    "bodies10": lambda: _planets_only(10),
 }
 
 
 @pytest.mark.parametrize("name,nsteps", [("cfg1", 300), ("cfg4", 50), ("star_only", 60), ("bodies4", 40),
-                                         ("bodies6", 40)])
+                                         ("bodies6", 40), ("bodies9", 40), ("bodies10", 40)])
 def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps):
     """Star + planets only: the multi-step kernel equals one launch per kernel
-    bit for bit, for 1 to 6 massive bodies (the kernel is specialised for 1-4)."""
+    bit for bit, for 1 to 10 massive bodies (the kernel is specialised for 1-9)."""
     w = FUSED[name]()
     st = w.state.copy()
     # one grain inside Rmin, one exactly on a planet: removal must agree too
