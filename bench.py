@@ -3,7 +3,10 @@
 Default workload: the self-gravitating swarm (1 Msun star + 65,536 bodies,
 masses log-uniform in [1e-10, 1e-7] Msun, a~U(20,40) AU, Plummer eps = 1e-3 AU,
 dt = 0.02 yr) integrated with RK4 in the FP32-force / FP64-state form; the
-plain FP64 form runs the same steps for the speed ratio.  A "step" is one RK4
+plain FP64 form runs the same steps for the speed ratio, and so does the FP64
+form with refined hardware reciprocal square roots in the swarm sums (FP64_RSQ,
+in "fp64_rsq"; skipped without a swarm, where it is the plain FP64
+computation).  A "step" is one RK4
 step (four force evaluations) of the whole system with the state resident in
 HBM.  Metric: directed pairwise interactions per second,
 n_grav (n_grav - 1) + n_dust n_grav per evaluation, four evaluations per step.
@@ -29,7 +32,7 @@ sys.path.insert(0, ROOT)
 
 from diskdyn_amd import synth  # noqa: E402
 from diskdyn_amd.engine import DDYN_KCLASS_DUST, DDYN_KCLASS_PAIRS, DDYN_KCLASS_STAGE, Engine  # noqa: E402
-from diskdyn_amd.system import FP32_FORCE, FP64  # noqa: E402
+from diskdyn_amd.system import FP32_FORCE, FP64, FP64_RSQ  # noqa: E402
 
 FLOP_PER_INTERACTION = 20
 PEAK_FP32_TFLOPS = 157.3
@@ -356,6 +359,13 @@ def main():
     k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
     t64, kt64, a64, drift64, _ = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
     print(f"[bench] {args.workload} plain FP64: {k64} steps in {t64:.3f} s", file=sys.stderr, flush=True)
+    ns = w32.state.n_grav - w32.params.n_planets
+    rsq = None
+    if ns > 0:
+        # FP64 with refined hardware reciprocal square roots in the swarm sums
+        # (without a swarm it is the plain FP64 computation)
+        trq, ktrq, arq, driftrq, _ = timed_run(w32, FP64_RSQ, k64, min(args.warmup, 2), args.in_order)
+        print(f"[bench] {args.workload} FP64_RSQ: {k64} steps in {trq:.3f} s", file=sys.stderr, flush=True)
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)
@@ -365,7 +375,6 @@ def main():
     # dominant kernel: massive pairs (swarm workloads) or the grain kernel (debris ring)
     dom = DDYN_KCLASS_PAIRS if w32.state.n_dust == 0 else DDYN_KCLASS_DUST
     nl, ms = kt32[dom]
-    ns = w32.state.n_grav - w32.params.n_planets
     if dom == DDYN_KCLASS_PAIRS:
         inter_launch = w32.state.n_grav * ns
         kname = "k_pairs32"
@@ -381,6 +390,16 @@ def main():
     nl64, ms64 = kt64[dom]
     inter64 = per_step * k64 / max(nl64, 1) if (ns == 0 and dom != DDYN_KCLASS_PAIRS) else inter_launch
     ach64 = FLOP_PER_INTERACTION * inter64 / (ms64 / max(nl64, 1) * 1e-3) / 1e12
+    if ns > 0:
+        vrq = per_step * k64 / trq
+        nlrq, msrq = ktrq[dom]
+        achrq = FLOP_PER_INTERACTION * inter_launch / (msrq / max(nlrq, 1) * 1e-3) / 1e12
+        medrq, maxrq = force_error(w32, arq)
+        rsq = {"value": vrq, "ms_per_step": 1e3 * trq / k64, "steps": k64,
+               "force_rel_err_median": medrq, "force_rel_err_max": maxrq,
+               "kernel_tflops": achrq, "kernel_frac_fp64_peak": achrq / PEAK_FP64_TFLOPS,
+               "speedup_fp32_vs_fp64_rsq": value / vrq, "energy_drift": driftrq,
+               "note": "FP64 throughout; 1/R^3 of the swarm sums from v_rsq_f64 + one third-order step"}
 
     out = {
         "metric": "pairwise interactions/s & RK4 steps/s, FP32-force vs FP64 at 1e-5 rel err",
@@ -404,6 +423,7 @@ def main():
                  "force_rel_err_median": med64, "force_rel_err_max": max64,
                  "kernel_tflops": ach64, "kernel_frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS},
         "speedup_fp32_vs_fp64": value / v64,
+        "fp64_rsq": rsq,
         "energy_drift": {"fp32_force": drift32, "fp64": drift64,
                          "note": "|dE/E0| of the massive set over the timed steps (each form's own step count)"},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,

@@ -57,15 +57,32 @@ constexpr int kMaxBins = 2700;      // nsize, src/ddyn.cuh:35
 
 // ---------------------------------------------------------------------------
 // Massive swarm-pair kernels
+// 1/sqrt(x) for x > 0 in FP64 (the adjoint kernels and the swarm sums of the
+// DDYN_FP64_RSQ form): the hardware estimate v_rsq_f64 (relative error up to
+// 5.2e-8) refined by one third-order step y <- y + y e (1/2 + 3e/8) with
+// e = 1 - x y^2, the series of (1 - e)^(-1/2) to e^2.  The estimate's error
+// cubed is far below FP64 rounding: 1.24 ulp at most over 2^22 inputs spread
+// over 24 decades, as two Newton steps give, in five operations instead of
+// eight.  Cheaper than an IEEE square root and division.
+__device__ inline double rsqrt_fp64(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = __builtin_fma(-x * y, y, 1.0);
+    return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
+}
+
 // ---------------------------------------------------------------------------
 // Targets: every massive body i in [0, n_grav).  Sources: swarm bodies j in the
 // slice [n_planets + y*slice_len, ...).  Lane l of wave w handles targets
 // wbase + l + 64k, k < TPT, so each wave covers one contiguous index range.
 // Pair law (src/gpu_functions.cu:157-172): a_i += mu_j (r_j - r_i) / R^3 with
 // R^2 = |r_j - r_i|^2 + eps^2 for swarm targets (eps = 0 for star/planet targets),
-// and a zero contribution when R^3 == 0.
+// and a zero contribution when R^3 == 0.  RSQ (the DDYN_FP64_RSQ form) takes
+// 1/R^3 from rsqrt_fp64 instead of an IEEE square root and division, with R^2
+// clamped at 1e-200 so that a coincident pair (d = 0) contributes 0 * finite
+// without a select; only separations below 1e-100 AU are affected.
 
-template <int TPT>
+template <int TPT, bool RSQ>
 __global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos, int n_grav,
                                                  int n_planets, int slice_len, double eps2,
                                                  double* __restrict__ part)
@@ -99,8 +116,14 @@ __global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos
             for (int k = 0; k < TPT; ++k) {
                 const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
                 const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) + e2[k];
-                const double f = r2 * __builtin_sqrt(r2);
-                const double q = (f != 0.0) ? s.w / f : 0.0;
+                double q;
+                if (RSQ) {
+                    const double y = rsqrt_fp64(__builtin_fmax(r2, 1e-200));
+                    q = s.w * (y * y * y);
+                } else {
+                    const double f = r2 * __builtin_sqrt(r2);
+                    q = (f != 0.0) ? s.w / f : 0.0;
+                }
                 ax[k] = __builtin_fma(dx, q, ax[k]);
                 ay[k] = __builtin_fma(dy, q, ay[k]);
                 az[k] = __builtin_fma(dz, q, az[k]);
@@ -478,7 +501,7 @@ struct DustShared {
     int32_t flag[kDustSplit][64];    // per-wave "grain removed" flags
 };
 
-template <bool F32, bool ACCEL>
+template <bool F32, bool ACCEL, bool RSQ = false>
 __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWavesPerEU, 8))) void k_dust(
     DustArgs A)
 {
@@ -582,7 +605,13 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
                     const double RRu = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
                     hit = hit || (RRu == 0.0);
                     const double R2 = RRu + e2;
-                    const double f = sj.w / (R2 * __builtin_sqrt(R2));
+                    double f;
+                    if (RSQ) {
+                        const double y = rsqrt_fp64(R2);
+                        f = sj.w * (y * y * y);
+                    } else {
+                        f = sj.w / (R2 * __builtin_sqrt(R2));
+                    }
                     ax = __builtin_fma(-f, ex, ax);
                     ay = __builtin_fma(-f, ey, ay);
                     az = __builtin_fma(-f, ez, az);
@@ -894,19 +923,6 @@ __global__ __launch_bounds__(64) void k_small_commit(DustArgs A)
 //   grains:  star term differentiated in closed form (gravity, radiation
 //            pressure, PR and wind drag), pulls mu_j K(R_j - R_t).
 
-// 1/sqrt(x) for x > 0 in FP64 (the adjoint kernels): the hardware estimate
-// v_rsq_f64 refined by two Newton steps y <- y + y (1 - x y^2) / 2, which
-// leaves a few ulp of error; cheaper than an IEEE square root and division.
-__device__ inline double rsqrt_fp64(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const double r = __builtin_fma(-x * y, y, 1.0);
-        y = __builtin_fma(0.5 * y, r, y);
-    }
-    return y;
-}
 
 // Massive VJP, all pairs: targets k (one per lane), sources j in slice y of
 // [0, n_grav).  Writes part[(y*4 + c) * n + k], c = x, y, z, mu.
@@ -1596,9 +1612,12 @@ int launch_pairs(ddyn_ctx* c, int s)
         k_pairs32<1><<<grid, kWG, 0, c->stream>>>(c->p32[s], c->p64[s], ng, np, c->slice_len,
                                                    (float)eps2_of(c), (float)(r64 * r64),
                                                    eps2_of(c), c->part);
+    } else if (c->prm.precision == DDYN_FP64_RSQ) {
+        k_pairs64<1, true><<<grid, kWG, 0, c->stream>>>(c->p64[s], ng, np, c->slice_len, eps2_of(c),
+                                                         c->part);
     } else {
-        k_pairs64<1><<<grid, kWG, 0, c->stream>>>(c->p64[s], ng, np, c->slice_len, eps2_of(c),
-                                                   c->part);
+        k_pairs64<1, false><<<grid, kWG, 0, c->stream>>>(c->p64[s], ng, np, c->slice_len, eps2_of(c),
+                                                          c->part);
     }
     DDYN_HIP(c, hipGetLastError());
     return DDYN_OK;
@@ -1664,12 +1683,14 @@ int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel, hipStream_t st = nul
     if (!st) st = c->stream;
     TimedLaunch tl(c, DDYN_KCLASS_DUST, st);
     const unsigned blocks = (unsigned)((c->n_dust + 63) / 64);
-    const bool f32 = c->prm.precision == DDYN_FP32_FORCE;
+    const int form = c->prm.precision;
     if (accel) {
-        if (f32) k_dust<true, true><<<blocks, kDustWG, 0, st>>>(A);
+        if (form == DDYN_FP32_FORCE) k_dust<true, true><<<blocks, kDustWG, 0, st>>>(A);
+        else if (form == DDYN_FP64_RSQ) k_dust<false, true, true><<<blocks, kDustWG, 0, st>>>(A);
         else k_dust<false, true><<<blocks, kDustWG, 0, st>>>(A);
     } else {
-        if (f32) k_dust<true, false><<<blocks, kDustWG, 0, st>>>(A);
+        if (form == DDYN_FP32_FORCE) k_dust<true, false><<<blocks, kDustWG, 0, st>>>(A);
+        else if (form == DDYN_FP64_RSQ) k_dust<false, false, true><<<blocks, kDustWG, 0, st>>>(A);
         else k_dust<false, false><<<blocks, kDustWG, 0, st>>>(A);
     }
     DDYN_HIP(c, hipGetLastError());
@@ -1779,7 +1800,7 @@ int ddyn_set_params(ddyn_ctx* c, const ddyn_params* p)
 {
     if (!c || !p) return DDYN_ERR_ARG;
     if (p->n_planets < 1) return fail(c, DDYN_ERR_ARG, "n_planets must count the star (>= 1)");
-    if (p->precision != DDYN_FP64 && p->precision != DDYN_FP32_FORCE)
+    if (p->precision != DDYN_FP64 && p->precision != DDYN_FP32_FORCE && p->precision != DDYN_FP64_RSQ)
         return fail(c, DDYN_ERR_ARG, "unknown precision form");
     if (!(p->eps >= 0.0) || !(p->Rmin >= 0.0)) return fail(c, DDYN_ERR_ARG, "eps and Rmin must be >= 0");
     if (p->SWQ == 1 && !(p->vsw > 0.0)) return fail(c, DDYN_ERR_ARG, "SWQ needs vsw > 0");

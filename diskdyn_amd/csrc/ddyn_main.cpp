@@ -10,8 +10,9 @@
 // SEDs are not produced; their output times still clip the step so the
 // sequence of steps matches the reference.
 //
-// Options: --fp32 (FP32-force form), --eps <AU> (softening of swarm pairs),
-//          --rmin <AU> (sublimation radius), --device <n>, --quiet.
+// Options: --fp32 (FP32-force form), --fp64-rsq (FP64 with refined hardware
+//          reciprocal square roots in the swarm sums), --eps <AU> (softening of
+//          swarm pairs), --rmin <AU> (sublimation radius), --device <n>, --quiet.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,7 +32,7 @@ constexpr double kYear = 3.1556926e7;
 void usage()
 {
     std::fprintf(stderr,
-                 "Usage: ddyn -i <init.param> | -m <model.dat> [--fp32] [--eps AU] [--rmin AU] "
+                 "Usage: ddyn -i <init.param> | -m <model.dat> [--fp32 | --fp64-rsq] [--eps AU] [--rmin AU] "
                  "[--device n] [--quiet]\n");
 }
 
@@ -65,6 +66,7 @@ int main(int argc, char** argv)
         if (s == "-i" && a + 1 < argc) param = argv[++a];
         else if (s == "-m" && a + 1 < argc) model = argv[++a];
         else if (s == "--fp32") precision = DDYN_FP32_FORCE;
+        else if (s == "--fp64-rsq") precision = DDYN_FP64_RSQ;
         else if (s == "--eps" && a + 1 < argc) eps = std::atof(argv[++a]);
         else if (s == "--rmin" && a + 1 < argc) rmin = std::atof(argv[++a]);
         else if (s == "--device" && a + 1 < argc) device = std::atoi(argv[++a]);

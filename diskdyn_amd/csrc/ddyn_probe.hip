@@ -8,6 +8,10 @@
 //                      registers: alone, with its LDS reads, without v_min3,
 //                      with v_rsq replaced by v_mul, 4 / 8 / 16 interactions
 //                      per loop iteration
+//   ddyn_probe rsq64   accuracy of FP64 reciprocal square roots over 2^22
+//                      inputs spread over 24 decades: the v_rsq_f64 estimate
+//                      alone, refined by two Newton steps, and refined by one
+//                      third-order step, against long double on the host
 //   ddyn_probe copy    a plain copy kernel of known size (256 MiB read, 256 MiB
 //                      written, three launches) to calibrate the FETCH_SIZE and
 //                      WRITE_SIZE counters of rocprofv3 --pmc
@@ -15,6 +19,7 @@
 // Every kernel indexes only within its grid; buffers are sized to the grid.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -122,6 +127,64 @@ __global__ void r_sqrtdiv64(double* out, int iters)
     double s = 0.0;
     for (int k = 0; k < 8; ++k) s += a[k];
     out[t] = s;
+}
+
+// out[3i]: v_rsq_f64 estimate; out[3i+1]: two Newton steps y += y (1 - x y^2) / 2;
+// out[3i+2]: one third-order step y += y e (1/2 + 3e/8), e = 1 - x y^2 (the
+// form the library uses).
+__global__ __launch_bounds__(256) void r_rsq64(const double* __restrict__ in, double* __restrict__ out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = in[i];
+    const double y0 = __builtin_amdgcn_rsq(x);
+    double y = y0;
+    for (int it = 0; it < 2; ++it) {
+        const double r = __builtin_fma(-x * y, y, 1.0);
+        y = __builtin_fma(0.5 * y, r, y);
+    }
+    const double e = __builtin_fma(-x * y0, y0, 1.0);
+    const double h = __builtin_fma(y0 * e, __builtin_fma(0.375, e, 0.5), y0);
+    out[3 * i] = y0;
+    out[3 * i + 1] = y;
+    out[3 * i + 2] = h;
+}
+
+int rsq64()
+{
+    const int n = 1 << 22;
+    double* hin = (double*)std::malloc(sizeof(double) * n);
+    double* hout = (double*)std::malloc(sizeof(double) * 3 * n);
+    unsigned long long s = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < n; ++i) {
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        const double u = (double)(s >> 11) * (1.0 / 9007199254740992.0);
+        hin[i] = std::pow(10.0, -12.0 + 24.0 * u);
+    }
+    double *din, *dout;
+    CK(hipMalloc(&din, sizeof(double) * n));
+    CK(hipMalloc(&dout, sizeof(double) * 3 * n));
+    CK(hipMemcpy(din, hin, sizeof(double) * n, hipMemcpyHostToDevice));
+    r_rsq64<<<(n + 255) / 256, 256>>>(din, dout, n);
+    CK(hipGetLastError());
+    CK(hipMemcpy(hout, dout, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    const char* names[3] = {"v_rsq_f64 estimate", "two Newton steps", "one third-order step"};
+    for (int k = 0; k < 3; ++k) {
+        long double emax = 0.0L, esum2 = 0.0L;
+        for (int i = 0; i < n; ++i) {
+            const long double ref = 1.0L / std::sqrt((long double)hin[i]);
+            const long double e = std::fabs(((long double)hout[3 * i + k] - ref) / ref);
+            emax = e > emax ? e : emax;
+            esum2 += e * e;
+        }
+        std::printf("%-22s max rel err %.3Le (%.2Lf ulp of 2^-53), rms %.3Le\n", names[k], emax,
+                    emax / 1.1102230246251565e-16L, std::sqrt(esum2 / n));
+    }
+    CK(hipFree(din));
+    CK(hipFree(dout));
+    std::free(hin);
+    std::free(hout);
+    return 0;
 }
 
 __global__ void copy_kernel(const float4* __restrict__ a, float4* __restrict__ b, size_t n)
@@ -349,6 +412,7 @@ int main(int argc, char** argv)
     if (argc == 2 && std::strcmp(argv[1], "rates") == 0) return rates();
     if (argc == 2 && std::strcmp(argv[1], "copy") == 0) return copy();
     if (argc == 2 && std::strcmp(argv[1], "mix") == 0) return mix();
-    std::fprintf(stderr, "usage: ddyn_probe rates | copy | mix\n");
+    if (argc == 2 && std::strcmp(argv[1], "rsq64") == 0) return rsq64();
+    std::fprintf(stderr, "usage: ddyn_probe rates | copy | mix | rsq64\n");
     return 1;
 }

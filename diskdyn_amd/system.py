@@ -13,8 +13,9 @@ import numpy as np
 
 from .units import LIGHT_AU
 
-FP64 = 0
-FP32_FORCE = 1
+FP64 = 0          # every term in FP64, IEEE square roots and divisions
+FP32_FORCE = 1    # swarm-sourced pairs in FP32 tiles; everything else FP64
+FP64_RSQ = 2      # FP64; swarm-sourced pairs use refined hardware reciprocal square roots
 
 
 @dataclass
@@ -22,7 +23,7 @@ class Params:
     n_planets: int                 # star + planets (cfg->n_planets counts the star)
     SWQ: int = 0                   # stellar-wind drag on
     FLQ: int = 0                   # Lorentz force on
-    precision: int = FP64          # FP64 or FP32_FORCE
+    precision: int = FP64          # FP64, FP32_FORCE or FP64_RSQ
     eps: float = 0.0               # Plummer softening [AU] of swarm-sourced pairs
     Rmin: float = 0.0              # sublimation radius [AU]
     vsw: float = 0.0               # stellar-wind speed [AU/yr]

@@ -46,6 +46,9 @@ extern "C" {
 #define DDYN_FP64 0           /* every term in FP64 (the plain form) */
 #define DDYN_FP32_FORCE 1     /* swarm-sourced pairs in FP32 with FP64 per-tile flushes;
                                  star/planet terms, close pairs and RK4 state in FP64 */
+#define DDYN_FP64_RSQ 2       /* every term in FP64; swarm-sourced pairs take 1/R^3 from the
+                                 v_rsq_f64 estimate refined by one third-order step (1.24 ulp
+                                 at most) instead of an IEEE square root and division */
 
 typedef struct ddyn_ctx ddyn_ctx;
 
@@ -55,7 +58,7 @@ typedef struct {
     int32_t n_planets;   /* star + planets (cfg->n_planets counts the star, src/read_in.cu:63-79) */
     int32_t SWQ;         /* 1: stellar-wind drag on (.param record 10) */
     int32_t FLQ;         /* 1: Lorentz force on (.param record 10) */
-    int32_t precision;   /* DDYN_FP64 or DDYN_FP32_FORCE */
+    int32_t precision;   /* DDYN_FP64, DDYN_FP32_FORCE or DDYN_FP64_RSQ */
     double  eps;         /* Plummer softening [AU] for swarm-sourced pairs; 0 = reference law (none) */
     double  Rmin;        /* grain sublimation radius [AU] (Grid::Rmin, src/grid.cu:737) */
     double  vsw;         /* stellar-wind speed [AU/yr] (cfg->vsw*year/AU, src/grid.cu:737) */

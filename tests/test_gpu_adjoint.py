@@ -11,7 +11,7 @@ import torch
 
 from diskdyn_amd import engine, synth
 from diskdyn_amd.adjoint import Runner, Trajectory, planet_initial_state, ring_loss
-from diskdyn_amd.system import FP32_FORCE, FP64
+from diskdyn_amd.system import FP32_FORCE, FP64, FP64_RSQ
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -48,6 +48,26 @@ def test_gpu_adjoint_matches_cpu_adjoint(engine_cls, name):
     print("  ", {k: f"{v:.2e}" for k, v in errs.items()})
     assert max(errs.values()) <= 1e-9
     assert np.abs(fin.pos() - ref.pos()).max() <= 1e-10 * np.abs(ref.pos()).max()
+
+
+@pytest.mark.parametrize("name", ["debris", "swarm"])
+def test_gpu_adjoint_fp64_rsq_context(engine_cls, name):
+    """FP64_RSQ context: the forward recompute takes the refined-rsqrt swarm
+    sums; the gradients still meet the FP64 bar against the CPU adjoint."""
+    make, nsteps, ck = CASES[name]
+    w = make()
+    st = w.state
+    rng = np.random.default_rng(13)
+    lam_r, lam_v = rng.normal(size=(st.n_all, 3)), rng.normal(size=(st.n_all, 3))
+    with engine_cls() as eng:
+        eng.upload(st.copy(), w.params.with_precision(FP64_RSQ))
+        g_r, g_v, g_mu, g_beta = eng.adjoint(w.dt, nsteps, lam_r, lam_v, ckpt_every=ck)
+    c_r, c_v, c_mu, c_beta = O.adjoint(w.params, st, w.dt, nsteps, lam_r, lam_v)
+    errs = {"r": _rel(g_r, c_r), "v": _rel(g_v, c_v), "mu": _rel(g_mu, c_mu)}
+    if st.n_dust:
+        errs["beta"] = _rel(g_beta, c_beta)
+    print(f"[adjoint fp64_rsq] {name}", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) <= 1e-9
 
 
 def test_gpu_adjoint_fp32_tiles(engine_cls):

@@ -7,13 +7,15 @@ what the dynamics fixes whatever the size:
   from both sides, so only rounding breaks it);
 - total energy and angular momentum drift only by the RK4 truncation error;
 - the FP32-force and plain FP64 forms stay together (the FP32 error is a small
-  perturbation of the swarm term, far below the star's pull).
+  perturbation of the swarm term, far below the star's pull), and the FP64 form
+  with refined hardware reciprocal square roots stays with the plain one to
+  rounding.
 """
 import numpy as np
 import pytest
 
 from diskdyn_amd import synth
-from diskdyn_amd.system import FP32_FORCE, FP64
+from diskdyn_amd.system import FP32_FORCE, FP64, FP64_RSQ
 from tests.paritylib import pos_rel_err
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +37,7 @@ def momentum_scale(st):
     return float(np.sum(st.mu[:ng] * np.sqrt(st.vx[:ng] ** 2 + st.vy[:ng] ** 2 + st.vz[:ng] ** 2)))
 
 
-@pytest.mark.parametrize("precision,bound", [(FP64, 1e-13), (FP32_FORCE, 1e-9)])
+@pytest.mark.parametrize("precision,bound", [(FP64, 1e-13), (FP32_FORCE, 1e-9), (FP64_RSQ, 1e-13)])
 def test_swarm_65537_conservation(engine_cls, precision, bound):
     w = synth.config2(precision=precision)
     print(f"[fullsize] swarm n_grav={w.state.n_grav} precision={precision}", flush=True)
@@ -52,12 +54,16 @@ def test_swarm_65537_conservation(engine_cls, precision, bound):
 
 
 def test_debris_ring_forms_agree(engine_cls):
-    """262,144 grains + 1,026 massive bodies, 20 steps in both forms."""
+    """262,144 grains + 1,026 massive bodies, 20 steps in all three forms."""
     w = synth.config3(precision=FP64)
     print(f"[fullsize] debris ring n_dust={w.state.n_dust}", flush=True)
     s64, e0, e1 = run(engine_cls, w, FP64, 20)
     s32, _, _ = run(engine_cls, w, FP32_FORCE, 20)
-    assert np.array_equal(s64.N, s32.N)
+    srq, _, _ = run(engine_cls, w, FP64_RSQ, 20)
+    assert np.array_equal(s64.N, s32.N) and np.array_equal(s64.N, srq.N)
+    erq = pos_rel_err(srq, s64)
+    print(f"  FP64_RSQ vs FP64 positions: max {erq.max():.2e}")
+    assert erq.max() <= 1e-12
     err = pos_rel_err(s32, s64)
     print(f"  FP32-force vs FP64 positions: median {np.median(err):.2e} max {err.max():.2e}; "
           f"|dE/E| {abs((e1[0] - e0[0]) / e0[0]):.2e}")
@@ -67,11 +73,15 @@ def test_debris_ring_forms_agree(engine_cls):
 
 
 def test_disk_1048577_forms_agree(engine_cls):
-    """One RK4 step of the 1,048,577-body disk in both forms."""
+    """One RK4 step of the 1,048,577-body disk in all three forms."""
     w = synth.config5(precision=FP64)
     print(f"[fullsize] disk n_grav={w.state.n_grav}", flush=True)
     s64, e0, e1 = run(engine_cls, w, FP64, 1)
     s32, _, _ = run(engine_cls, w, FP32_FORCE, 1)
+    srq, _, _ = run(engine_cls, w, FP64_RSQ, 1)
+    erq = pos_rel_err(srq, s64)
+    print(f"  FP64_RSQ vs FP64 positions: max {erq.max():.2e}")
+    assert erq.max() <= 1e-12
     err = pos_rel_err(s32, s64)
     dP = np.abs(e1[4:7] - e0[4:7]).max() / momentum_scale(w.state)
     print(f"  FP32-force vs FP64 positions: median {np.median(err):.2e} max {err.max():.2e}; FP64 |dP| {dP:.2e}")

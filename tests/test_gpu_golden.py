@@ -11,14 +11,14 @@ from dataclasses import replace
 import numpy as np
 import pytest
 
-from diskdyn_amd.system import FP32_FORCE, FP64
+from diskdyn_amd.system import FP32_FORCE, FP64, FP64_RSQ
 from oracle import oracle as O
 from tests.paritylib import golden_state, pos_rel_err, rel_err_rows
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", [FP64, FP32_FORCE])
+@pytest.mark.parametrize("precision", [FP64, FP32_FORCE, FP64_RSQ])
 def test_gpu_golden_accelerations(engine_cls, precision):
     st, prm, g = golden_state()
     with engine_cls() as eng:
@@ -27,7 +27,7 @@ def test_gpu_golden_accelerations(engine_cls, precision):
     assert not rem.any()
     err = rel_err_rows(a, g["acc"])
     print(f"[golden] precision={precision} accel max rel {err.max():.2e} median {np.median(err):.2e}")
-    if precision == FP64:
+    if precision in (FP64, FP64_RSQ):
         assert err.max() <= 1e-13
     else:
         assert np.median(err) <= 1e-5

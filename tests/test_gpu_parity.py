@@ -2,7 +2,8 @@
 
 Bars: plain FP64 form <= 1e-12 relative on per-particle accelerations and
 <= 1e-10 relative on positions after the stated steps; FP32-force form
-<= 1e-5 median relative acceleration error.
+<= 1e-5 median relative acceleration error; the FP64 form with refined hardware
+reciprocal square roots (FP64_RSQ) is held to the plain FP64 bars.
 """
 from dataclasses import replace
 
@@ -10,7 +11,7 @@ import numpy as np
 import pytest
 
 from diskdyn_amd import synth
-from diskdyn_amd.system import FP32_FORCE, FP64
+from diskdyn_amd.system import FP32_FORCE, FP64, FP64_RSQ
 from oracle import oracle as O
 from tests.paritylib import pos_rel_err, rel_err_rows, subset, vel_rel_err
 
@@ -45,6 +46,22 @@ def test_accel_fp64(engine_cls, name):
 
 
 @pytest.mark.parametrize("name", ["cfg2", "cfg3", "cfg5"])
+def test_accel_fp64_rsq(engine_cls, name):
+    """Refined hardware reciprocal square roots in the swarm sums: the plain FP64
+    bar, and not the plain form's bits (the form reaches the kernels)."""
+    w = SMALL[name]()
+    print(f"[accel_fp64_rsq] {name}", flush=True)
+    a_gpu, rem_gpu = _gpu_accel(engine_cls, w, FP64_RSQ)
+    a_plain, _ = _gpu_accel(engine_cls, w, FP64)
+    a_ref, rem_ref = O.accel(w.params, w.state)
+    err = rel_err_rows(a_gpu, a_ref)
+    print(f"  max rel {err.max():.3e} median {np.median(err):.3e}")
+    assert np.array_equal(rem_gpu, rem_ref)
+    assert err.max() <= 1e-12
+    assert not np.array_equal(a_gpu, a_plain)
+
+
+@pytest.mark.parametrize("name", ["cfg2", "cfg3", "cfg5"])
 def test_accel_fp32(engine_cls, name):
     w = SMALL[name]()
     print(f"[accel_fp32] {name}", flush=True)
@@ -65,6 +82,24 @@ def test_step_fp64(engine_cls, name, nsteps):
     st_gpu = w.state.copy()
     with engine_cls() as eng:
         eng.upload(st_gpu, w.params.with_precision(FP64))
+        eng.step(w.dt, nsteps)
+        eng.download(st_gpu)
+    st_ref = O.step(w.params, w.state.copy(), w.dt, nsteps)
+    ep = pos_rel_err(st_gpu, st_ref)
+    ev = vel_rel_err(st_gpu, st_ref)
+    print(f"  pos max {ep.max():.3e} vel max {ev.max():.3e}")
+    assert np.array_equal(st_gpu.N, st_ref.N)
+    assert ep.max() <= 1e-10
+    assert ev.max() <= 1e-9
+
+
+@pytest.mark.parametrize("name,nsteps", [("cfg2", 10), ("cfg3", 20), ("cfg5", 4)])
+def test_step_fp64_rsq(engine_cls, name, nsteps):
+    w = SMALL[name]()
+    print(f"[step_fp64_rsq] {name} steps={nsteps}", flush=True)
+    st_gpu = w.state.copy()
+    with engine_cls() as eng:
+        eng.upload(st_gpu, w.params.with_precision(FP64_RSQ))
         eng.step(w.dt, nsteps)
         eng.download(st_gpu)
     st_ref = O.step(w.params, w.state.copy(), w.dt, nsteps)
@@ -164,7 +199,7 @@ def test_fused_small_steps_more_workgroups_than_fit_at_once(engine_cls):
     assert pos_rel_err(out[0], ref, slice(0, w.state.n_grav)).max() <= 1e-10
 
 
-@pytest.mark.parametrize("precision", [FP64, FP32_FORCE])
+@pytest.mark.parametrize("precision", [FP64, FP32_FORCE, FP64_RSQ])
 def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision):
     """Swarm + grains: the grain kernel of step k beside the massive stages of
     step k+1 (double-buffered stage positions) equals one launch per kernel in
