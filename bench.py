@@ -240,11 +240,12 @@ def adjoint_run(name, precision=FP32_FORCE):
           L = mean |r(T) - r_target(T)|^2 with the target from planet
           m = 1.2e-4 Msun, a = 26 AU; gradients in planet (m, a, e) checked by
           central finite differences (h = 1e-6 relative).
-    cfg5: 1,048,577-body disk, 10-step window; FP32-force form with
-          checkpoints every 5, or the plain FP64 form (--fp64) with checkpoints
-          every 2; L = mean (|r(T)| - |r_Kepler(T)|)^2; directional derivative
-          in the initial state checked by central differences.  Progress of
-          each forward chunk and adjoint window goes to stderr."""
+    cfg5: 1,048,577-body disk, 10-step window in one tape segment; FP32-force
+          form, or the plain FP64 form (--fp64), or FP64 with refined hardware
+          reciprocal square roots (--fp64-rsq); L = mean (|r(T)| - |r_Kepler(T)|)^2;
+          directional derivative in the initial state checked by central
+          differences.  Progress of each forward chunk and adjoint window goes
+          to stderr."""
     import torch
 
     from diskdyn_amd.adjoint import (Runner, Trajectory, kepler_deviation_loss, kepler_targets,
@@ -318,8 +319,9 @@ def adjoint_run(name, precision=FP32_FORCE):
             fd = (loss(r0 + h * dr, v0 + h * dv, mu).item() - loss(r0 - h * dr, v0 - h * dv, mu).item()) / (2 * h)
         an = float((r0.grad * dr).sum() + (v0.grad * dv).sum())
         out.update({"value": nsteps / t_adj, "seconds": t_adj, "steps": nsteps, "ckpt_every": ck,
-                    "dtype": ("fp32 pair forces and VJP tiles, fp64 state" if precision == FP32_FORCE
-                              else "fp64"), "loss": L.item(),
+                    "dtype": {FP32_FORCE: "fp32 pair forces and VJP tiles, fp64 state",
+                              FP64_RSQ: "fp64, refined hardware rsqrt in the swarm sums"}.get(precision, "fp64"),
+                    "loss": L.item(),
                     "directional": {"adjoint": an, "fd": fd, "rel_err": abs(an - fd) / abs(fd)},
                     "config": {"workload": "cfg5_large_disk 10-step adjoint window: 1 Msun star + 1,048,576 bodies, "
                                "eps=5e-4 AU, dt=0.02 yr", "n_grav": st.n_grav, "n_dust": 0}})
@@ -339,6 +341,8 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
     ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
+    ap.add_argument("--fp64-rsq", action="store_true",
+                    help="with --adjoint cfg5: FP64 with refined hardware rsqrt in the swarm sums")
     ap.add_argument("--in-order", action="store_true",
                     help="time every pass with one launch per kernel, in order (kernel profiling runs)")
     args = ap.parse_args()
@@ -349,7 +353,7 @@ def main():
     if args.adjoint:
         if args.workload not in ("cfg4", "cfg5"):
             raise SystemExit("--adjoint runs the cfg4 or cfg5 workload")
-        return adjoint_run(args.workload, FP64 if args.fp64 else FP32_FORCE)
+        return adjoint_run(args.workload, FP64 if args.fp64 else FP64_RSQ if args.fp64_rsq else FP32_FORCE)
 
     w32, desc = workload(args.workload, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
