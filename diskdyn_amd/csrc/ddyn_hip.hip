@@ -924,8 +924,37 @@ __global__ __launch_bounds__(64) void k_small_commit(DustArgs A)
 //            pressure, PR and wind drag), pulls mu_j K(R_j - R_t).
 
 
+// One FP64 pair of the massive VJP: target k (position and mass pk, Abar_k),
+// source j (pj, Abar_j), softening e2 of this pair.  Adds
+// J(d) (mu_k Abar_j - mu_j Abar_k) to g and -Abar_j . K(d) to gm, with
+// d = r_j - r_k, s^2 = |d|^2 + e2, K(d) = d / s^3 and J(d) w = w / s^3 - 3 d (d.w) / s^5.
+// w is formed from plain products, so a softened self pair (j == k, d = 0)
+// has w = 0 exactly and contributes nothing; s^2 = 0 (an unsoftened self or
+// coincident pair) contributes nothing either.
+__device__ inline void vjp_pair_fp64(const double4 pk, double akx, double aky, double akz, const double4 pj,
+                                     double ajx, double ajy, double ajz, double e2, double& gx, double& gy,
+                                     double& gz, double& gm)
+{
+    const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
+    const double s2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, e2)));
+    if (s2 > 0.0) {
+        const double inv = rsqrt_fp64(s2);
+        const double inv2 = inv * inv, inv3 = inv2 * inv, inv5 = inv3 * inv2;
+        const double wx = pk.w * ajx - pj.w * akx, wy = pk.w * ajy - pj.w * aky, wz = pk.w * ajz - pj.w * akz;
+        const double dw = __builtin_fma(dz, wz, __builtin_fma(dy, wy, dx * wx));
+        const double t3 = 3.0 * dw * inv5;
+        gx = __builtin_fma(wx, inv3, __builtin_fma(-dx, t3, gx));
+        gy = __builtin_fma(wy, inv3, __builtin_fma(-dy, t3, gy));
+        gz = __builtin_fma(wz, inv3, __builtin_fma(-dz, t3, gz));
+        const double da = __builtin_fma(ajz, dz, __builtin_fma(ajy, dy, ajx * dx));
+        gm = __builtin_fma(-da, inv3, gm);
+    }
+}
+
 // Massive VJP, all pairs: targets k (one per lane), sources j in slice y of
-// [0, n_grav).  Writes part[(y*4 + c) * n + k], c = x, y, z, mu.
+// [0, n_grav).  Writes part[(y*4 + c) * n + k], c = x, y, z, mu.  The LDS tile
+// carries each source's softening flag (1 for swarm sources, 0 for the star
+// and planets) so that a pair's softening is e2k * flag.
 __global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ pos,
                                                    const double* __restrict__ Ab, int n_grav,
                                                    int n_planets, int slice_len, double eps2,
@@ -948,26 +977,12 @@ __global__ __launch_bounds__(kWG) void k_vjp_pairs(const double4* __restrict__ p
         if ((int)threadIdx.x < tl) {
             const int j = j0 + threadIdx.x;
             tp[threadIdx.x] = pos[j];
-            ta[threadIdx.x] = make_double4(Ab[j], Ab[n + j], Ab[2 * n + j], 0.0);
+            ta[threadIdx.x] = make_double4(Ab[j], Ab[n + j], Ab[2 * n + j], (j >= n_planets) ? 1.0 : 0.0);
         }
         __syncthreads();
         for (int jj = 0; jj < tl; ++jj) {
-            const int j = j0 + jj;
-            const double4 pj = tp[jj];
             const double4 aj = ta[jj];
-            const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
-            const double s2 = dx * dx + dy * dy + dz * dz + ((j >= n_planets) ? e2k : 0.0);
-            if (s2 > 0.0) {
-                const double inv = rsqrt_fp64(s2);
-                const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
-                const double wx = pk.w * aj.x - pj.w * akx, wy = pk.w * aj.y - pj.w * aky,
-                             wz = pk.w * aj.z - pj.w * akz;
-                const double dw = dx * wx + dy * wy + dz * wz;
-                gx += wx * inv3 - 3.0 * dx * dw * inv5;
-                gy += wy * inv3 - 3.0 * dy * dw * inv5;
-                gz += wz * inv3 - 3.0 * dz * dw * inv5;
-                gm -= (aj.x * dx + aj.y * dy + aj.z * dz) * inv3;
-            }
+            vjp_pair_fp64(pk, akx, aky, akz, tp[jj], aj.x, aj.y, aj.z, e2k * aj.w, gx, gy, gz, gm);
         }
     }
     if (ok) {
@@ -982,22 +997,9 @@ __device__ inline void vjp_tile_fp64(const double4* __restrict__ pos, const doub
                                      int j0, int j1, int n_planets, double e2k, const double4 pk, double akx,
                                      double aky, double akz, double& gx, double& gy, double& gz, double& gm)
 {
-    for (int j = j0; j < j1; ++j) {
-        const double4 pj = pos[j];
-        const double ajx = Ab[j], ajy = Ab[n + j], ajz = Ab[2 * n + j];
-        const double dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
-        const double s2 = dx * dx + dy * dy + dz * dz + ((j >= n_planets) ? e2k : 0.0);
-        if (s2 > 0.0) {
-            const double inv = rsqrt_fp64(s2);
-            const double inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
-            const double wx = pk.w * ajx - pj.w * akx, wy = pk.w * ajy - pj.w * aky, wz = pk.w * ajz - pj.w * akz;
-            const double dw = dx * wx + dy * wy + dz * wz;
-            gx += wx * inv3 - 3.0 * dx * dw * inv5;
-            gy += wy * inv3 - 3.0 * dy * dw * inv5;
-            gz += wz * inv3 - 3.0 * dz * dw * inv5;
-            gm -= (ajx * dx + ajy * dy + ajz * dz) * inv3;
-        }
-    }
+    for (int j = j0; j < j1; ++j)
+        vjp_pair_fp64(pk, akx, aky, akz, pos[j], Ab[j], Ab[n + j], Ab[2 * n + j], (j >= n_planets) ? e2k : 0.0,
+                      gx, gy, gz, gm);
 }
 
 // FP32 VJP sum over one LDS tile for one target; with SELF the source slot
