@@ -177,7 +177,7 @@ int rsq64()
             emax = e > emax ? e : emax;
             esum2 += e * e;
         }
-        std::printf("%-22s max rel err %.3Le (%.2Lf ulp of 2^-53), rms %.3Le\n", names[k], emax,
+        std::printf("%-22s max rel err %.3Le (%.2Lf x 2^-53), rms %.3Le\n", names[k], emax,
                     emax / 1.1102230246251565e-16L, std::sqrt(esum2 / n));
     }
     CK(hipFree(din));

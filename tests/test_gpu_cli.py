@@ -91,3 +91,20 @@ def test_param_with_generated_particles_is_refused(tmp_path):
     p.write_text(PARAM.replace("-3.5 0.1 -1.0 0 0", "-3.5 0.1 -1.0 0 100"))
     r = subprocess.run([DDYN, "-i", str(p), "--quiet"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "not supported" in r.stderr
+
+
+def test_precision_flags_without_swarm_give_the_same_file(tmp_path):
+    """Star and planets only: the three precision forms are the same
+    computation, so --fp32 and --fp64-rsq write the default run's file byte
+    for byte (and the flags parse)."""
+    p = tmp_path / "sys.param"
+    p.write_text(PARAM_NO_DISKS.replace("20.0\n200\n", "1.0\n40\n").replace("5.0 10.0 10.0", "1.0 1.0 1.0"))
+    texts = []
+    for flag in ([], ["--fp32"], ["--fp64-rsq"]):
+        d = tmp_path / ("run" + "".join(flag).replace("-", "_"))
+        d.mkdir()
+        r = subprocess.run([DDYN, "-i", str(p), "--quiet", *flag], cwd=d, capture_output=True, text=True, timeout=120)
+        print(flag, r.returncode, r.stderr)
+        assert r.returncode == 0
+        texts.append((d / "teststub_1.00.dat").read_text())
+    assert texts[1] == texts[0] and texts[2] == texts[0]
