@@ -1343,6 +1343,71 @@ __global__ void k_adj_mstage(AdjArgs A, int with_grains)
     }
 }
 
+// The massive part of one reverse step when n_grav <= kWG, in one workgroup
+// with one body per thread: k_adj_init, then for stages 3..0 the VJP pair sums
+// and the stage chain rule of k_adj_mstage.  Each stage's Abar of every body
+// goes through LDS to the next stage's pair sums.  The operations and their
+// order are those of the separate launches with one source slice (where the
+// FP32-force form redoes its only tile, which holds the star, in FP64), so the
+// result is the same bit for bit; nine launches per reverse step become one.
+__global__ __launch_bounds__(kWG) void k_adj_massive_small(AdjArgs A, int with_grains)
+{
+    __shared__ double4 sp[kWG];   // stage positions and masses
+    __shared__ double4 sa[kWG];   // stage Abar and the softening flag
+    const int i = threadIdx.x;
+    const int ng = A.n_grav;
+    const bool ok = i < ng;
+    const size_t n = (size_t)ng, na = A.n_all;
+    const double h = A.dt;
+    const double w[4] = {h / 6.0, h / 3.0, h / 3.0, h / 6.0};
+    const double hh[3] = {h / 2.0, h / 2.0, h};
+    const double e2i = (i >= A.n_planets) ? A.eps2 : 0.0;
+    double rb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, Vb[4][3], Ab[4][3], gmu = 0.0;
+    if (ok) {
+        for (int c = 0; c < 3; ++c) { rb[c] = A.rb[c * na + i]; vb[c] = A.vb[c * na + i]; }
+        gmu = A.g_mu[i];
+    }
+    for (int st = 0; st < 4; ++st)
+        for (int c = 0; c < 3; ++c) { Vb[st][c] = w[st] * rb[c]; Ab[st][c] = w[st] * vb[c]; }
+#pragma unroll
+    for (int ST = 3; ST >= 0; --ST) {
+        __syncthreads();   // the previous stage's readers of sp and sa are done
+        if (ok) {
+            sp[i] = A.p64[ST][i];
+            sa[i] = make_double4(Ab[ST][0], Ab[ST][1], Ab[ST][2], (i >= A.n_planets) ? 1.0 : 0.0);
+        }
+        __syncthreads();
+        if (!ok) continue;
+        const double4 pk = sp[i];
+        double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
+        for (int j = 0; j < ng; ++j) {
+            const double4 aj = sa[j];
+            vjp_pair_fp64(pk, Ab[ST][0], Ab[ST][1], Ab[ST][2], sp[j], aj.x, aj.y, aj.z, e2i * aj.w, gx, gy, gz,
+                          gm);
+        }
+        double Rb[3] = {0.0, 0.0, 0.0}, mb = 0.0;
+        Rb[0] += gx; Rb[1] += gy; Rb[2] += gz; mb += gm;
+        if (with_grains) {
+            for (int c = 0; c < 3; ++c) Rb[c] += A.Rbg[((size_t)ST * 3 + c) * n + i];
+            if (ST == 3) mb += A.mug[i];
+        }
+        gmu += mb;
+        for (int c = 0; c < 3; ++c) {
+            const double V = Vb[ST][c];
+            rb[c] += Rb[c];
+            vb[c] += V;
+            if (ST > 0) {
+                Vb[ST - 1][c] += hh[ST - 1] * Rb[c];
+                Ab[ST - 1][c] += hh[ST - 1] * V;
+            }
+        }
+    }
+    if (ok) {
+        for (int c = 0; c < 3; ++c) { A.rb[c * na + i] = rb[c]; A.vb[c * na + i] = vb[c]; }
+        A.g_mu[i] = gmu;
+    }
+}
+
 // Copy of the full state (x, y, z, vx, vy, vz, N: 7 blocks of n_all).
 __global__ void k_copy_state(size_t n_all, const double* x, const double* y, const double* z,
                              const double* vx, const double* vy, const double* vz, const double* N,
@@ -2148,12 +2213,19 @@ int reverse_tape(ddyn_ctx* c, const double* lam_r, const double* lam_v, double* 
         for (int64_t k = k1 - 1; k >= k0; --k) {
             restore_state(c, c->tape_seg + (size_t)(k - k0) * 7 * na);
             for (int st = 1; st <= 3; ++st) A.p64[st] = c->tape_stg + ((size_t)(k - k0) * 3 + (size_t)(st - 1)) * ng;
-            k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
+            // star + planets and few other massive bodies: one launch for the massive part
+            const bool small = ng <= (size_t)kWG && c->fuse_small;
+            if (!small) k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
             if (nd) {
                 k_adj_dust<<<(unsigned)((nd + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
                 k_adj_reduce<<<(unsigned)((4 * ng + kWG / 64 - 1) / (kWG / 64)), kWG, 0, c->stream>>>(A);
             }
             const int wg = nd ? 1 : 0;
+            if (small) {
+                k_adj_massive_small<<<1, kWG, 0, c->stream>>>(A, wg);
+                DDYN_HIP(c, hipGetLastError());
+                continue;
+            }
             for (int st = 3; st >= 0; --st) {
                 if (c->prm.precision == DDYN_FP32_FORCE) {
                     const double r64 = r64_of(c);

@@ -5,6 +5,8 @@ Bars: GPU adjoint == CPU adjoint to 1e-9 relative (same algebra, different
 summation order); gradients within 1e-4 relative of central finite
 differences with h = 1e-6 relative.
 """
+from dataclasses import replace
+
 import numpy as np
 import pytest
 import torch
@@ -68,6 +70,30 @@ def test_gpu_adjoint_fp64_rsq_context(engine_cls, name):
         errs["beta"] = _rel(g_beta, c_beta)
     print(f"[adjoint fp64_rsq] {name}", {k: f"{v:.2e}" for k, v in errs.items()})
     assert max(errs.values()) <= 1e-9
+
+
+@pytest.mark.parametrize("name,precision", [("ring", FP64), ("debris", FP64), ("debris", FP32_FORCE),
+                                            ("debris", FP64_RSQ), ("swarm250", FP32_FORCE), ("swarm250", FP64)])
+def test_fused_small_massive_reverse_equals_per_stage_launches(engine_cls, name, precision):
+    """At most 256 massive bodies: the single-workgroup massive reverse step
+    (schedule 0) gives the gradients of the per-stage launches (schedule 1)
+    bit for bit, with and without grains, in every precision form."""
+    if name == "swarm250":
+        make, nsteps, ck = (lambda: synth.config2(n_swarm=250, precision=FP64, eps=1e-2)), 6, 4
+    else:
+        make, nsteps, ck = CASES[name]
+    w = make()
+    st = w.state
+    rng = np.random.default_rng(14)
+    lam_r, lam_v = rng.normal(size=(st.n_all, 3)), rng.normal(size=(st.n_all, 3))
+    out = []
+    for schedule in (0, 1):
+        with engine_cls() as eng:
+            eng.upload(st.copy(), replace(w.params, precision=precision, schedule=schedule))
+            out.append(eng.adjoint(w.dt, nsteps, lam_r, lam_v, ckpt_every=ck))
+    print(f"[fused reverse] {name} precision={precision} n_grav={st.n_grav} n_dust={st.n_dust}", flush=True)
+    for a, b in zip(out[0], out[1]):
+        assert (a is None and b is None) or np.array_equal(a, b)
 
 
 def test_gpu_adjoint_fp32_tiles(engine_cls):
