@@ -72,16 +72,20 @@ def test_gpu_adjoint_fp64_rsq_context(engine_cls, name):
     assert max(errs.values()) <= 1e-9
 
 
+FUSED_REVERSE = {
+    "swarm250": (lambda: synth.config2(n_swarm=250, precision=FP64, eps=1e-2), 6, 4),
+    "debris10": (lambda: synth.config3(n_dust=300, n_swarm=8, precision=FP64, eps=2e-2), 8, 3),
+}
+
+
 @pytest.mark.parametrize("name,precision", [("ring", FP64), ("debris", FP64), ("debris", FP32_FORCE),
-                                            ("debris", FP64_RSQ), ("swarm250", FP32_FORCE), ("swarm250", FP64)])
+                                            ("debris", FP64_RSQ), ("swarm250", FP32_FORCE), ("swarm250", FP64),
+                                            ("debris10", FP64), ("debris10", FP32_FORCE)])
 def test_fused_small_massive_reverse_equals_per_stage_launches(engine_cls, name, precision):
     """At most 256 massive bodies: the single-workgroup massive reverse step
     (schedule 0) gives the gradients of the per-stage launches (schedule 1)
     bit for bit, with and without grains, in every precision form."""
-    if name == "swarm250":
-        make, nsteps, ck = (lambda: synth.config2(n_swarm=250, precision=FP64, eps=1e-2)), 6, 4
-    else:
-        make, nsteps, ck = CASES[name]
+    make, nsteps, ck = FUSED_REVERSE[name] if name in FUSED_REVERSE else CASES[name]
     w = make()
     st = w.state
     rng = np.random.default_rng(14)
