@@ -1219,21 +1219,24 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
             const double m0 = s0.w;
             const double d[3] = {Rs[0] - s0.x, Rs[1] - s0.y, Rs[2] - s0.z};
             const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            const double Rr = __builtin_sqrt(R2), R3 = R2 * Rr, R4 = R2 * R2, R5 = R3 * R2, R6 = R4 * R2;
+            // powers of 1/R from one refined reciprocal square root: the
+            // Jacobian below needs 1/R^2 .. 1/R^6 about thirty times per stage
+            const double i1 = rsqrt_fp64(R2), i2 = i1 * i1, i3 = i2 * i1, i4 = i2 * i2, i5 = i4 * i1,
+                         i6 = i4 * i2;
             const double Gc = (1.0 - bet - betsw) * m0;
             const double dv = d[0] * Vs[0] + d[1] * Vs[1] + d[2] * Vs[2];
             const double dA = d[0] * Av[0] + d[1] * Av[1] + d[2] * Av[2];
             const double vA = Vs[0] * Av[0] + Vs[1] * Av[1] + Vs[2] * Av[2];
             double db[3], t2A = 0.0;
             for (int c = 0; c < 3; ++c) {
-                const double J0A = Av[c] / R3 - 3.0 * d[c] * dA / R5;
-                db[c] = -Gc * J0A - m0 * kap * ((Vs[c] * dA + dv * Av[c]) / R4 - 4.0 * dv * dA * d[c] / R6 -
-                                                 2.0 * d[c] * vA / R4);
-                Vb[st][c] += act ? -m0 * kap * (d[c] * dA / R4 + Av[c] / R2) : 0.0;
-                t2A += (dv * d[c] / R4 + Vs[c] / R2) * Av[c];
+                const double J0A = Av[c] * i3 - 3.0 * d[c] * dA * i5;
+                db[c] = -Gc * J0A - m0 * kap * ((Vs[c] * dA + dv * Av[c]) * i4 - 4.0 * dv * dA * d[c] * i6 -
+                                                 2.0 * d[c] * vA * i4);
+                Vb[st][c] += act ? -m0 * kap * (d[c] * dA * i4 + Av[c] * i2) : 0.0;
+                t2A += (dv * d[c] * i4 + Vs[c] * i2) * Av[c];
             }
-            const double bb = m0 * dA / R3 - m0 * dkap * t2A;
-            const double mb = -(1.0 - bet - betsw) * dA / R3 - kap * t2A;
+            const double bb = m0 * dA * i3 - m0 * dkap * t2A;
+            const double mb = -(1.0 - bet - betsw) * dA * i3 - kap * t2A;
             double c4[4];
             for (int c = 0; c < 3; ++c) {
                 if (act) Rb[c] += db[c];
