@@ -4,9 +4,8 @@ Default workload: the self-gravitating swarm (1 Msun star + 65,536 bodies,
 masses log-uniform in [1e-10, 1e-7] Msun, a~U(20,40) AU, Plummer eps = 1e-3 AU,
 dt = 0.02 yr) integrated with RK4 in the FP32-force / FP64-state form; the
 plain FP64 form runs the same steps for the speed ratio, and so does the FP64
-form with refined hardware reciprocal square roots in the swarm sums (FP64_RSQ,
-in "fp64_rsq"; skipped without a swarm, where it is the plain FP64
-computation).  A "step" is one RK4
+form with refined hardware reciprocal square roots in every pair law (FP64_RSQ,
+in "fp64_rsq").  A "step" is one RK4
 step (four force evaluations) of the whole system with the state resident in
 HBM.  Metric: directed pairwise interactions per second,
 n_grav (n_grav - 1) + n_dust n_grav per evaluation, four evaluations per step.
@@ -369,12 +368,9 @@ def main():
     t64, kt64, a64, drift64, _ = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
     print(f"[bench] {args.workload} plain FP64: {k64} steps in {t64:.3f} s", file=sys.stderr, flush=True)
     ns = w32.state.n_grav - w32.params.n_planets
-    rsq = None
-    if ns > 0:
-        # FP64 with refined hardware reciprocal square roots in the swarm sums
-        # (without a swarm it is the plain FP64 computation)
-        trq, ktrq, arq, driftrq, _ = timed_run(w32, FP64_RSQ, k64, min(args.warmup, 2), args.in_order)
-        print(f"[bench] {args.workload} FP64_RSQ: {k64} steps in {trq:.3f} s", file=sys.stderr, flush=True)
+    # FP64 with refined hardware reciprocal square roots in every pair law
+    trq, ktrq, arq, driftrq, _ = timed_run(w32, FP64_RSQ, k64, min(args.warmup, 2), args.in_order)
+    print(f"[bench] {args.workload} FP64_RSQ: {k64} steps in {trq:.3f} s", file=sys.stderr, flush=True)
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)
@@ -399,16 +395,16 @@ def main():
     nl64, ms64 = kt64[dom]
     inter64 = per_step * k64 / max(nl64, 1) if (ns == 0 and dom != DDYN_KCLASS_PAIRS) else inter_launch
     ach64 = FLOP_PER_INTERACTION * inter64 / (ms64 / max(nl64, 1) * 1e-3) / 1e12
-    if ns > 0:
-        vrq = per_step * k64 / trq
-        nlrq, msrq = ktrq[dom]
-        achrq = FLOP_PER_INTERACTION * inter_launch / (msrq / max(nlrq, 1) * 1e-3) / 1e12
-        medrq, maxrq = force_error(w32, arq)
-        rsq = {"value": vrq, "ms_per_step": 1e3 * trq / k64, "steps": k64,
-               "force_rel_err_median": medrq, "force_rel_err_max": maxrq,
-               "kernel_tflops": achrq, "kernel_frac_fp64_peak": achrq / PEAK_FP64_TFLOPS,
-               "speedup_fp32_vs_fp64_rsq": value / vrq, "energy_drift": driftrq,
-               "note": "FP64 throughout; 1/R^3 of the swarm sums from v_rsq_f64 + one third-order step"}
+    vrq = per_step * k64 / trq
+    nlrq, msrq = ktrq[dom]
+    interrq = per_step * k64 / max(nlrq, 1) if (ns == 0 and dom != DDYN_KCLASS_PAIRS) else inter_launch
+    achrq = FLOP_PER_INTERACTION * interrq / (msrq / max(nlrq, 1) * 1e-3) / 1e12
+    medrq, maxrq = force_error(w32, arq)
+    rsq = {"value": vrq, "ms_per_step": 1e3 * trq / k64, "steps_per_s": k64 / trq, "steps": k64,
+           "force_rel_err_median": medrq, "force_rel_err_max": maxrq,
+           "kernel_tflops": achrq, "kernel_frac_fp64_peak": achrq / PEAK_FP64_TFLOPS,
+           "speedup_fp32_vs_fp64_rsq": value / vrq, "speedup_vs_plain_fp64": vrq / v64, "energy_drift": driftrq,
+           "note": "FP64 throughout; 1/R^3 (and the star term's 1/R^2) from v_rsq_f64 + one third-order step"}
 
     out = {
         "metric": "pairwise interactions/s & RK4 steps/s, FP32-force vs FP64 at 1e-5 rel err",

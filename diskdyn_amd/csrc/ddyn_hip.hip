@@ -71,16 +71,29 @@ __device__ inline double rsqrt_fp64(double x)
     return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
 }
 
+// mu / R^3 of an unsoftened (or softened) pair with R^2 = r2, 0 when R^3 = 0:
+// an IEEE square root and division (the plain form), or rsqrt_fp64 with R^2
+// clamped at 1e-200 (DDYN_FP64_RSQ), where a coincident pair (d = 0) still
+// contributes d * q = 0 without a select; only separations below 1e-100 AU
+// are affected.
+template <bool RSQ>
+__device__ inline double pair_q(double r2, double mu)
+{
+    if (RSQ) {
+        const double y = rsqrt_fp64(__builtin_fmax(r2, 1e-200));
+        return mu * (y * y * y);
+    }
+    const double f = r2 * __builtin_sqrt(r2);
+    return (f != 0.0) ? mu / f : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Targets: every massive body i in [0, n_grav).  Sources: swarm bodies j in the
 // slice [n_planets + y*slice_len, ...).  Lane l of wave w handles targets
 // wbase + l + 64k, k < TPT, so each wave covers one contiguous index range.
 // Pair law (src/gpu_functions.cu:157-172): a_i += mu_j (r_j - r_i) / R^3 with
 // R^2 = |r_j - r_i|^2 + eps^2 for swarm targets (eps = 0 for star/planet targets),
-// and a zero contribution when R^3 == 0.  RSQ (the DDYN_FP64_RSQ form) takes
-// 1/R^3 from rsqrt_fp64 instead of an IEEE square root and division, with R^2
-// clamped at 1e-200 so that a coincident pair (d = 0) contributes 0 * finite
-// without a select; only separations below 1e-100 AU are affected.
+// and a zero contribution when R^3 == 0 (pair_q; RSQ: the DDYN_FP64_RSQ form).
 
 template <int TPT, bool RSQ>
 __global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos, int n_grav,
@@ -116,14 +129,7 @@ __global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos
             for (int k = 0; k < TPT; ++k) {
                 const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
                 const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) + e2[k];
-                double q;
-                if (RSQ) {
-                    const double y = rsqrt_fp64(__builtin_fmax(r2, 1e-200));
-                    q = s.w * (y * y * y);
-                } else {
-                    const double f = r2 * __builtin_sqrt(r2);
-                    q = (f != 0.0) ? s.w / f : 0.0;
-                }
+                const double q = pair_q<RSQ>(r2, s.w);
                 ax[k] = __builtin_fma(dx, q, ax[k]);
                 ay[k] = __builtin_fma(dy, q, ay[k]);
                 az[k] = __builtin_fma(dz, q, az[k]);
@@ -311,8 +317,9 @@ __device__ inline void store_pos(double4* p64, float4* p32, int i, double x, dou
     p32[i] = make_float4((float)x, (float)y, (float)z, (float)mu);
 }
 
-// STAGE 0..3: RK4 stage s; STAGE 4: write accelerations only.
-template <int STAGE>
+// STAGE 0..3: RK4 stage s; STAGE 4: write accelerations only.  RSQ: the
+// DDYN_FP64_RSQ form of the direct terms (pair_q).
+template <int STAGE, bool RSQ>
 __global__ __launch_bounds__(kWG) void k_stage(StageArgs A)
 {
     const int i = blockIdx.x * kWG + threadIdx.x;
@@ -325,8 +332,7 @@ __global__ __launch_bounds__(kWG) void k_stage(StageArgs A)
         const double4 pj = P[j];
         const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        const double f = r2 * __builtin_sqrt(r2);
-        const double q = (f != 0.0) ? pj.w / f : 0.0;
+        const double q = pair_q<RSQ>(r2, pj.w);
         ax = __builtin_fma(dx, q, ax);
         ay = __builtin_fma(dy, q, ay);
         az = __builtin_fma(dz, q, az);
@@ -431,24 +437,37 @@ __device__ inline GrainCoef grain_coef(const DustArgs& A, double bet, double bet
 
 // Star term with radiation pressure, PR and stellar-wind drag and the Lorentz
 // force (src/gpu_functions.cu:679-718).  Returns true if the grain sublimates.
+template <bool RSQ>
 __device__ inline bool star_term(const DustArgs& A, const double4 s0, double px, double py,
                                  double pz, double ux, double uy, double uz, const GrainCoef& k,
                                  double& ax, double& ay, double& az)
 {
     const double dx = px - s0.x, dy = py - s0.y, dz = pz - s0.z;
     const double RR = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-    const double RR3 = RR * __builtin_sqrt(RR);
     if (RR <= A.Rmin2) return true;
     const double m0 = s0.w;
-    const double Fg = k.gfac * m0 / RR3;
-    const double Fb1 = (A.SWQ == 1) ? m0 / RR * k.drag : k.drag / RR;
-    const double Fb2 = __builtin_fma(dz, uz, __builtin_fma(dy, uy, dx * ux)) / RR;
+    const double du = __builtin_fma(dz, uz, __builtin_fma(dy, uy, dx * ux));
+    double RR3 = 0.0, iR2 = 0.0, iR3 = 0.0, Fg, Fb1, Fb2;
+    if (RSQ) {   // DDYN_FP64_RSQ: powers of one refined 1/R
+        const double i1 = rsqrt_fp64(RR);
+        iR2 = i1 * i1;
+        iR3 = iR2 * i1;
+        Fg = k.gfac * m0 * iR3;
+        Fb1 = (A.SWQ == 1) ? m0 * iR2 * k.drag : k.drag * iR2;
+        Fb2 = du * iR2;
+    } else {
+        RR3 = RR * __builtin_sqrt(RR);
+        Fg = k.gfac * m0 / RR3;
+        Fb1 = (A.SWQ == 1) ? m0 / RR * k.drag : k.drag / RR;
+        Fb2 = du / RR;
+    }
     const double Fdx = Fb1 * __builtin_fma(Fb2, dx, ux);
     const double Fdy = Fb1 * __builtin_fma(Fb2, dy, uy);
     const double Fdz = Fb1 * __builtin_fma(Fb2, dz, uz);
     double Flx = 0.0, Fly = 0.0;
     if (A.FLQ == 1) {
-        const double Bz = A.Bstar * A.Rstar * A.Rstar * A.Rstar / RR3;
+        const double B3 = A.Bstar * A.Rstar * A.Rstar * A.Rstar;
+        const double Bz = RSQ ? B3 * iR3 : B3 / RR3;
         Flx = k.qe * __builtin_fma(-A.Wz, dx, uy) * Bz / k.m;
         Fly = -k.qe * __builtin_fma(A.Wz, dy, ux) * Bz / k.m;
     }
@@ -460,6 +479,7 @@ __device__ inline bool star_term(const DustArgs& A, const double4 s0, double px,
 
 // FP64 pull of sources [j0, j1) on the grain; `soft` adds eps^2.  Sets `hit`
 // on an exact coincidence (src/gpu_functions.cu:727-730).
+template <bool RSQ>
 __device__ inline void grain_fp64(const double4* __restrict__ P, int j0, int j1, double px,
                                   double py, double pz, double e2, double& ax, double& ay,
                                   double& az, bool& hit)
@@ -470,7 +490,13 @@ __device__ inline void grain_fp64(const double4* __restrict__ P, int j0, int j1,
         const double RRu = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
         hit = hit || (RRu == 0.0);
         const double R2 = RRu + e2;
-        const double f = s.w / (R2 * __builtin_sqrt(R2));
+        double f;
+        if (RSQ) {
+            const double y = rsqrt_fp64(R2);
+            f = s.w * (y * y * y);
+        } else {
+            f = s.w / (R2 * __builtin_sqrt(R2));
+        }
         ax = __builtin_fma(-f, ex, ax);
         ay = __builtin_fma(-f, ey, ay);
         az = __builtin_fma(-f, ez, az);
@@ -555,9 +581,9 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
         if (wave == 0 && act) {
             // star and planets (src/gpu_functions.cu:679-730), FP64
             const GrainCoef k = {S.coef[1][lane], S.coef[2][lane], S.coef[3][lane], S.coef[0][lane]};
-            hit = star_term(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane], k, ax, ay,
+            hit = star_term<RSQ>(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane], k, ax, ay,
                             az);
-            grain_fp64(A.p64[s], 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
+            grain_fp64<RSQ>(A.p64[s], 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
         }
         if (__builtin_amdgcn_readfirstlane(__any(act)) && jb < je) {
             if (F32) {
@@ -591,7 +617,7 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
                         if (!redo) {
                             ax += (double)fx; ay += (double)fy; az += (double)fz;
                         } else {
-                            grain_fp64(A.p64[s], j0, j0 + tl, px, py, pz, A.eps2, ax, ay, az, hit);
+                            grain_fp64<false>(A.p64[s], j0, j0 + tl, px, py, pz, A.eps2, ax, ay, az, hit);
                         }
                     }
                 }
@@ -706,6 +732,7 @@ constexpr int kSmallChunk = 1024;   // steps per launch of k_small_steps
 
 // Pair sum of the star and planets on massive body i at one stage
 // (src/gpu_functions.cu:157-172, as in k_stage).
+template <bool RSQ>
 __device__ inline void small_massive_accel(const double4* __restrict__ P, int i, int np, double& ax,
                                            double& ay, double& az)
 {
@@ -715,8 +742,7 @@ __device__ inline void small_massive_accel(const double4* __restrict__ P, int i,
         const double4 pj = P[j];
         const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        const double f = r2 * __builtin_sqrt(r2);
-        const double q = (f != 0.0) ? pj.w / f : 0.0;
+        const double q = pair_q<RSQ>(r2, pj.w);
         ax = __builtin_fma(dx, q, ax);
         ay = __builtin_fma(dy, q, ay);
         az = __builtin_fma(dz, q, az);
@@ -736,7 +762,7 @@ __device__ inline void wave_lds_sync()
 // and up to eight planets), so the pair loops unroll and their independent
 // square-root and division chains overlap (the sums keep their order, so every
 // NP gives the bits of NP = 0); NP = 0 takes it from A.
-template <int NP>
+template <int NP, bool RSQ>
 __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps)
 {
     __shared__ double4 P[2][4][kSmallMax];
@@ -761,7 +787,7 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
         for (int s = 0; s < 3; ++s) {
             if (body) {
                 double ax, ay, az;
-                small_massive_accel(P[buf][s], lane, np, ax, ay, az);
+                small_massive_accel<RSQ>(P[buf][s], lane, np, ax, ay, az);
                 const double h = hs[s];
                 const double ux = s ? vrx : mvx, uy = s ? vry : mvy, uz = s ? vrz : mvz;
                 const double rx = __builtin_fma(ax, h, mvx), ry = __builtin_fma(ay, h, mvy),
@@ -820,7 +846,7 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
             // massive stage 3 = final combination of this step (k_stage<3>) ...
             if (body) {
                 double ax, ay, az;
-                small_massive_accel(P[buf][3], lane, np, ax, ay, az);
+                small_massive_accel<RSQ>(P[buf][3], lane, np, ax, ay, az);
                 mx = __builtin_fma(dt6, Srx, mx); my = __builtin_fma(dt6, Sry, my);
                 mz = __builtin_fma(dt6, Srz, mz);
                 mvx = __builtin_fma(dt6, Sax + ax, mvx);
@@ -837,9 +863,9 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
             bool gone = false;
             for (int s = 0; s < 4 && !gone; ++s) {
                 double ax = 0.0, ay = 0.0, az = 0.0;
-                gone = star_term(A, Ps[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
+                gone = star_term<RSQ>(A, Ps[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
                 bool hit = false;
-                grain_fp64(Ps[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
+                grain_fp64<RSQ>(Ps[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
                 gone = gone || hit;
                 kx[s] = ax; ky[s] = ay; kz[s] = az;
                 if (s == 0) {
@@ -1180,11 +1206,11 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (live && !gone) {
             const double4* P = A.p64[st];
-            gone = star_term(D, P[0], R[st][0], R[st][1], R[st][2], V[st][0], V[st][1], V[st][2],
+            gone = star_term<false>(D, P[0], R[st][0], R[st][1], R[st][2], V[st][0], V[st][1], V[st][2],
                              grain_coef(D, bet, betsw, 0.0, m, P[0].w), ax, ay, az);
             bool hit = false;
-            grain_fp64(P, 1, A.n_planets, R[st][0], R[st][1], R[st][2], 0.0, ax, ay, az, hit);
-            grain_fp64(P, A.n_planets, A.n_grav, R[st][0], R[st][1], R[st][2], A.eps2, ax, ay, az, hit);
+            grain_fp64<false>(P, 1, A.n_planets, R[st][0], R[st][1], R[st][2], 0.0, ax, ay, az, hit);
+            grain_fp64<false>(P, A.n_planets, A.n_grav, R[st][0], R[st][1], R[st][2], A.eps2, ax, ay, az, hit);
             gone = gone || hit;
         }
         if (st == 3) break;
@@ -1713,12 +1739,22 @@ int launch_stage(ddyn_ctx* c, int s, const StageArgs& A)
 {
     TimedLaunch tl(c, DDYN_KCLASS_STAGE);
     const unsigned blocks = (unsigned)((c->n_grav + kWG - 1) / kWG);
-    switch (s) {
-        case 0: k_stage<0><<<blocks, kWG, 0, c->stream>>>(A); break;
-        case 1: k_stage<1><<<blocks, kWG, 0, c->stream>>>(A); break;
-        case 2: k_stage<2><<<blocks, kWG, 0, c->stream>>>(A); break;
-        case 3: k_stage<3><<<blocks, kWG, 0, c->stream>>>(A); break;
-        default: k_stage<4><<<blocks, kWG, 0, c->stream>>>(A); break;
+    if (c->prm.precision == DDYN_FP64_RSQ) {
+        switch (s) {
+            case 0: k_stage<0, true><<<blocks, kWG, 0, c->stream>>>(A); break;
+            case 1: k_stage<1, true><<<blocks, kWG, 0, c->stream>>>(A); break;
+            case 2: k_stage<2, true><<<blocks, kWG, 0, c->stream>>>(A); break;
+            case 3: k_stage<3, true><<<blocks, kWG, 0, c->stream>>>(A); break;
+            default: k_stage<4, true><<<blocks, kWG, 0, c->stream>>>(A); break;
+        }
+    } else {
+        switch (s) {
+            case 0: k_stage<0, false><<<blocks, kWG, 0, c->stream>>>(A); break;
+            case 1: k_stage<1, false><<<blocks, kWG, 0, c->stream>>>(A); break;
+            case 2: k_stage<2, false><<<blocks, kWG, 0, c->stream>>>(A); break;
+            case 3: k_stage<3, false><<<blocks, kWG, 0, c->stream>>>(A); break;
+            default: k_stage<4, false><<<blocks, kWG, 0, c->stream>>>(A); break;
+        }
     }
     DDYN_HIP(c, hipGetLastError());
     return DDYN_OK;
@@ -1994,17 +2030,32 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
         for (int64_t k = 0; k < nsteps; k += kSmallChunk) {
             const int n = (int)std::min<int64_t>(kSmallChunk, nsteps - k);
             TimedLaunch tl(c, DDYN_KCLASS_DUST);
-            switch (A.n_planets) {
-            case 1: k_small_steps<1><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 2: k_small_steps<2><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 3: k_small_steps<3><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 4: k_small_steps<4><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 5: k_small_steps<5><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 6: k_small_steps<6><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 7: k_small_steps<7><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 8: k_small_steps<8><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            case 9: k_small_steps<9><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-            default: k_small_steps<0><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+            if (c->prm.precision == DDYN_FP64_RSQ) {
+                switch (A.n_planets) {
+                case 1: k_small_steps<1, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 2: k_small_steps<2, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 3: k_small_steps<3, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 4: k_small_steps<4, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 5: k_small_steps<5, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 6: k_small_steps<6, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 7: k_small_steps<7, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 8: k_small_steps<8, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 9: k_small_steps<9, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                default: k_small_steps<0, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                }
+            } else {
+                switch (A.n_planets) {
+                case 1: k_small_steps<1, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 2: k_small_steps<2, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 3: k_small_steps<3, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 4: k_small_steps<4, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 5: k_small_steps<5, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 6: k_small_steps<6, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 7: k_small_steps<7, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 8: k_small_steps<8, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                case 9: k_small_steps<9, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                default: k_small_steps<0, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
+                }
             }
             DDYN_HIP(c, hipGetLastError());
             k_small_commit<<<1, 64, 0, c->stream>>>(A);

@@ -34,12 +34,12 @@ def test_gpu_golden_accelerations(engine_cls, precision):
         assert err.max() <= 1e-3
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
-def test_gpu_golden_rk4_step(engine_cls, schedule):
+@pytest.mark.parametrize("schedule,precision", [(0, FP64), (1, FP64), (0, FP64_RSQ), (1, FP64_RSQ)])
+def test_gpu_golden_rk4_step(engine_cls, schedule, precision):
     st, prm, g = golden_state()
     s = st.copy()
     with engine_cls() as eng:
-        eng.upload(s, replace(prm, schedule=schedule))
+        eng.upload(s, replace(prm, schedule=schedule, precision=precision))
         eng.step(float(g["step_dt"][0]), 1)
         eng.download(s)
     ep = rel_err_rows(s.pos(), g["pos1"])

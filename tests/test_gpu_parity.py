@@ -93,7 +93,7 @@ def test_step_fp64(engine_cls, name, nsteps):
     assert ev.max() <= 1e-9
 
 
-@pytest.mark.parametrize("name,nsteps", [("cfg2", 10), ("cfg3", 20), ("cfg5", 4)])
+@pytest.mark.parametrize("name,nsteps", [("cfg1", 100), ("cfg2", 10), ("cfg3", 20), ("cfg4", 40), ("cfg5", 4)])
 def test_step_fp64_rsq(engine_cls, name, nsteps):
     w = SMALL[name]()
     print(f"[step_fp64_rsq] {name} steps={nsteps}", flush=True)
@@ -146,11 +146,15 @@ FUSED = {
 }
 
 
-@pytest.mark.parametrize("name,nsteps", [("cfg1", 300), ("cfg4", 50), ("star_only", 60), ("bodies4", 40),
-                                         ("bodies6", 40), ("bodies9", 40), ("bodies10", 40)])
-def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps):
+@pytest.mark.parametrize("name,nsteps,precision", [("cfg1", 300, FP64), ("cfg4", 50, FP64), ("star_only", 60, FP64),
+                                                   ("bodies4", 40, FP64), ("bodies6", 40, FP64), ("bodies9", 40, FP64),
+                                                   ("bodies10", 40, FP64), ("cfg1", 300, FP64_RSQ),
+                                                   ("cfg4", 50, FP64_RSQ), ("bodies9", 40, FP64_RSQ),
+                                                   ("bodies10", 40, FP64_RSQ)])
+def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps, precision):
     """Star + planets only: the multi-step kernel equals one launch per kernel
-    bit for bit, for 1 to 10 massive bodies (the kernel is specialised for 1-9)."""
+    bit for bit, for 1 to 10 massive bodies (the kernel is specialised for 1-9),
+    in the plain FP64 form and in FP64_RSQ."""
     w = FUSED[name]()
     st = w.state.copy()
     # one grain inside Rmin, one exactly on a planet: removal must agree too
@@ -158,7 +162,7 @@ def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps):
     st.x[g0], st.y[g0], st.z[g0] = st.x[0] + 0.01, st.y[0], st.z[0]
     if st.n_grav > 1:
         st.x[g0 + 1], st.y[g0 + 1], st.z[g0 + 1] = st.x[1], st.y[1], st.z[1]
-    prm = replace(w.params, Rmin=0.05)
+    prm = replace(w.params, Rmin=0.05, precision=precision)
     out = []
     for schedule in (0, 1):
         s = st.copy()
