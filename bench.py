@@ -324,7 +324,7 @@ def adjoint_run(name, precision=FP32_FORCE):
         an = float((r0.grad * dr).sum() + (v0.grad * dv).sum())
         out.update({"value": nsteps / t_adj, "seconds": t_adj, "steps": nsteps, "ckpt_every": ck,
                     "dtype": {FP32_FORCE: "fp32 pair forces and VJP tiles, fp64 state",
-                              FP64_RSQ: "fp64, refined hardware rsqrt in the swarm sums"}.get(precision, "fp64"),
+                              FP64_RSQ: "fp64, refined hardware rsqrt in every pair law"}.get(precision, "fp64"),
                     "loss": L.item(),
                     "directional": {"adjoint": an, "fd": fd, "rel_err": abs(an - fd) / abs(fd)},
                     "config": {"workload": "cfg5_large_disk 10-step adjoint window: 1 Msun star + 1,048,576 bodies, "
@@ -346,7 +346,7 @@ def main():
                     help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
     ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
     ap.add_argument("--fp64-rsq", action="store_true",
-                    help="with --adjoint cfg5: FP64 with refined hardware rsqrt in the swarm sums")
+                    help="with --adjoint cfg5: FP64 with refined hardware rsqrt in every pair law")
     ap.add_argument("--in-order", action="store_true",
                     help="time every pass with one launch per kernel, in order (kernel profiling runs)")
     args = ap.parse_args()

@@ -57,7 +57,7 @@ constexpr int kMaxBins = 2700;      // nsize, src/ddyn.cuh:35
 
 // ---------------------------------------------------------------------------
 // Massive swarm-pair kernels
-// 1/sqrt(x) for x > 0 in FP64 (the adjoint kernels and the swarm sums of the
+// 1/sqrt(x) for x > 0 in FP64 (the adjoint kernels and every pair law of the
 // DDYN_FP64_RSQ form): the hardware estimate v_rsq_f64 (relative error up to
 // 5.2e-8) refined by one third-order step y <- y + y e (1/2 + 3e/8) with
 // e = 1 - x y^2, the series of (1 - e)^(-1/2) to e^2.  The estimate's error

@@ -11,7 +11,7 @@
 // sequence of steps matches the reference.
 //
 // Options: --fp32 (FP32-force form), --fp64-rsq (FP64 with refined hardware
-//          reciprocal square roots in the swarm sums), --eps <AU> (softening of
+//          reciprocal square roots in every pair law), --eps <AU> (softening of
 //          swarm pairs), --rmin <AU> (sublimation radius), --device <n>, --quiet.
 #include <cmath>
 #include <cstdio>

@@ -15,7 +15,7 @@ from .units import LIGHT_AU
 
 FP64 = 0          # every term in FP64, IEEE square roots and divisions
 FP32_FORCE = 1    # swarm-sourced pairs in FP32 tiles; everything else FP64
-FP64_RSQ = 2      # FP64; swarm-sourced pairs use refined hardware reciprocal square roots
+FP64_RSQ = 2      # FP64; every pair law uses refined hardware reciprocal square roots
 
 
 @dataclass

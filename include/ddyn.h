@@ -46,10 +46,10 @@ extern "C" {
 #define DDYN_FP64 0           /* every term in FP64 (the plain form) */
 #define DDYN_FP32_FORCE 1     /* swarm-sourced pairs in FP32 with FP64 per-tile flushes;
                                  star/planet terms, close pairs and RK4 state in FP64 */
-#define DDYN_FP64_RSQ 2       /* every term in FP64; swarm-sourced pairs take 1/R^3 from the
-                                 v_rsq_f64 estimate refined by one third-order step (relative
-                                 error at most 1.4e-16) instead of an IEEE square root and
-                                 division */
+#define DDYN_FP64_RSQ 2       /* every term in FP64; every pair law takes 1/R^3 (and the grain
+                                 star term 1/R^2) from the v_rsq_f64 estimate refined by one
+                                 third-order step (relative error at most 1.4e-16) instead of
+                                 an IEEE square root and division */
 
 typedef struct ddyn_ctx ddyn_ctx;
 

@@ -54,8 +54,8 @@ def test_gpu_adjoint_matches_cpu_adjoint(engine_cls, name):
 
 @pytest.mark.parametrize("name", ["debris", "swarm"])
 def test_gpu_adjoint_fp64_rsq_context(engine_cls, name):
-    """FP64_RSQ context: the forward recompute takes the refined-rsqrt swarm
-    sums; the gradients still meet the FP64 bar against the CPU adjoint."""
+    """FP64_RSQ context: the forward pass takes the refined-rsqrt pair
+    laws; the gradients still meet the FP64 bar against the CPU adjoint."""
     make, nsteps, ck = CASES[name]
     w = make()
     st = w.state

@@ -3,7 +3,8 @@
 Bars: plain FP64 form <= 1e-12 relative on per-particle accelerations and
 <= 1e-10 relative on positions after the stated steps; FP32-force form
 <= 1e-5 median relative acceleration error; the FP64 form with refined hardware
-reciprocal square roots (FP64_RSQ) is held to the plain FP64 bars.
+reciprocal square roots in every pair law (FP64_RSQ) is held to the plain FP64
+bars.
 """
 from dataclasses import replace
 
@@ -47,7 +48,7 @@ def test_accel_fp64(engine_cls, name):
 
 @pytest.mark.parametrize("name", ["cfg2", "cfg3", "cfg5"])
 def test_accel_fp64_rsq(engine_cls, name):
-    """Refined hardware reciprocal square roots in the swarm sums: the plain FP64
+    """Refined hardware reciprocal square roots in every pair law: the plain FP64
     bar, and not the plain form's bits (the form reaches the kernels)."""
     w = SMALL[name]()
     print(f"[accel_fp64_rsq] {name}", flush=True)
