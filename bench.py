@@ -1,20 +1,24 @@
 """Throughput of the dynamical hot path on one MI355X.
 
-Default workload: the self-gravitating swarm (1 Msun star + 65,536 bodies,
-masses log-uniform in [1e-10, 1e-7] Msun, a~U(20,40) AU, Plummer eps = 1e-3 AU,
-dt = 0.02 yr) integrated with RK4 in the FP32-force / FP64-state form; the
+Default workload (the configuration the metric is quoted on): the dusty debris
+ring, a 0.5 Msun / 0.1 Lsun star, a 3e-5 Msun planet at 20 AU, 1,024
+gravitating planetesimals (1e-9 Msun, Plummer eps = 1e-3 AU) and 262,144
+massless grains with radiation pressure, Poynting-Robertson and stellar-wind
+drag, dt = 0.05 yr, 500 RK4 steps, in the FP32-force / FP64-state form.  The
 plain FP64 form runs the same steps for the speed ratio, and so does the FP64
-form with refined hardware reciprocal square roots in every pair law (FP64_RSQ,
-in "fp64_rsq").  A "step" is one RK4
-step (four force evaluations) of the whole system with the state resident in
-HBM.  Metric: directed pairwise interactions per second,
-n_grav (n_grav - 1) + n_dust n_grav per evaluation, four evaluations per step.
+form with refined hardware reciprocal square roots in every pair law
+(FP64_RSQ, in "fp64_rsq").  A "step" is one RK4 step (four force evaluations)
+of the whole system with the state resident in HBM.  Metric: directed pairwise
+interactions per second, n_grav (n_grav - 1) + n_dust n_grav per evaluation,
+four evaluations per step.
 
-The roofline entry is the dominant kernel (the massive swarm-pair kernel, or
-the grain kernel where grains dominate): 20 FLOP per directed interaction
-(FMA = 2, rsqrt = 1) over its mean launch time measured with HIP events,
-against the FP32 vector peak.  The CPU
-baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample.
+The roofline entry is the dominant kernel (the grain kernel on the debris
+ring, the massive swarm-pair kernel on the swarms): 20 FLOP per directed
+interaction (FMA = 2, rsqrt = 1) over its mean launch time measured with HIP
+events on the stream it runs on, against the FP32 vector peak.  The CPU
+baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample
+of whole RK4 steps.  The self-gravitating swarm (65,537 bodies) and the large
+disk (1,048,577 bodies) run too, at reduced step counts, under "also".
 
 Prints one JSON line.
 """
@@ -37,11 +41,15 @@ FLOP_PER_INTERACTION = 20
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# Algorithmic HBM bytes of the four massive stage epilogues (k_stage<0..3>) per
-# massive body per RK4 step: 200 + 272 + 248 + 224 B of state, stage positions
-# (FP64 + FP32 shadow) and RK4 sums, plus 3 FP64 slice partials read per stage.
-STAGE_BYTES_PER_BODY = 944
-STAGE_PARTIAL_BYTES_PER_SLICE = 4 * 3 * 8
+# Algorithmic HBM bytes of the state update per RK4 step.  Massive bodies (the
+# four stage epilogues k_stage<0..3>): stages 1-3 read r0, v0 (48 B), read and
+# write the RK4 sums (2 x 48 B) and write the stage r, v (48 B), 192 B each;
+# stage 4 reads r0, v0 and the sums (96 B) and writes r, v (48 B), 144 B;
+# 720 B, plus the 16 B FP32 shadow of the position and mass at each of the four
+# stages.  Grains (fused into the grain kernel): read r, v (48 B), N (8 B) and
+# per-grain beta, beta_sw (16 B), write r, v (48 B): 120 B.
+STAGE_BYTES_PER_BODY = 720 + 4 * 16
+GRAIN_BYTES_PER_GRAIN = 120
 
 
 def workload(name, precision):
@@ -121,18 +129,31 @@ def timed_run(w, precision, steps, warmup, in_order=False):
 
 
 def state_update(w, kt, steps):
-    """Achieved HBM bandwidth of the massive stage epilogues (the FP64 state
-    update) from their HIP-event times, or None where they do not run."""
+    """Achieved HBM bandwidth of the state update in algorithmic bytes: the
+    massive stage epilogues (k_stage<0..3>, 784 B per body per step) and, with
+    grains, the grain kernel's state reads and writes (120 B per grain per
+    step), each over its HIP-event time per step.  The epilogues also read the
+    pair kernels' slice partials; those bytes are the kernels' own scratch, not
+    state, and are not counted."""
+    out = {}
     nl, ms = kt[DDYN_KCLASS_STAGE]
-    if nl == 0:
-        return None
-    per_body = STAGE_BYTES_PER_BODY + STAGE_PARTIAL_BYTES_PER_SLICE * kt["nslice"]
-    nbytes = w.state.n_grav * per_body
-    ms_step = ms / steps
-    gbs = nbytes / (ms_step * 1e-3) / 1e9
-    return {"kernel": "k_stage<0..3>", "bytes_per_step": nbytes, "bytes_per_body": per_body,
-            "nslice": kt["nslice"], "ms_per_step": ms_step, "achieved_GBs": gbs,
-            "frac_hbm_peak": gbs / PEAK_HBM_GBS}
+    if nl:
+        nbytes = w.state.n_grav * STAGE_BYTES_PER_BODY
+        ms_step = ms / steps
+        gbs = nbytes / (ms_step * 1e-3) / 1e9
+        out["massive"] = {"kernel": "k_stage<0..3>", "bytes_per_body": STAGE_BYTES_PER_BODY,
+                          "bytes_per_step": nbytes, "ms_per_step": ms_step, "achieved_GBs": gbs,
+                          "frac_hbm_peak": gbs / PEAK_HBM_GBS, "nslice": kt["nslice"]}
+    nl, ms = kt[DDYN_KCLASS_DUST]
+    if nl and w.state.n_dust and w.state.n_grav > w.params.n_planets:
+        nbytes = w.state.n_dust * GRAIN_BYTES_PER_GRAIN
+        ms_step = ms / steps
+        gbs = nbytes / (ms_step * 1e-3) / 1e9
+        out["grains"] = {"kernel": "k_dust (fused with the grain forces)", "bytes_per_grain": GRAIN_BYTES_PER_GRAIN,
+                         "bytes_per_step": nbytes, "ms_per_step": ms_step, "achieved_GBs": gbs,
+                         "frac_hbm_peak": gbs / PEAK_HBM_GBS,
+                         "note": "VALU-bound kernel: the state is read and written once per step"}
+    return out or None
 
 
 def jacobi_drift(n_dust=4096, nsteps=1000):
@@ -333,49 +354,28 @@ def adjoint_run(name, precision=FP32_FORCE):
     return 0
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"])
-    ap.add_argument("--fp64-steps", type=int, default=None,
-                    help="steps for the plain FP64 comparison run (default: --steps)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--adjoint", action="store_true",
-                    help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
-    ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
-    ap.add_argument("--fp64-rsq", action="store_true",
-                    help="with --adjoint cfg5: FP64 with refined hardware rsqrt in every pair law")
-    ap.add_argument("--in-order", action="store_true",
-                    help="time every pass with one launch per kernel, in order (kernel profiling runs)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the swarm would shard by target block, the grains by grain range)", file=sys.stderr)
-        return 2
-    if args.adjoint:
-        if args.workload not in ("cfg4", "cfg5"):
-            raise SystemExit("--adjoint runs the cfg4 or cfg5 workload")
-        return adjoint_run(args.workload, FP64 if args.fp64 else FP64_RSQ if args.fp64_rsq else FP32_FORCE)
+DEFAULT_STEPS = {"cfg1": 1000, "cfg2": 200, "cfg3": 500, "cfg4": 400, "cfg5": 20}
 
-    w32, desc = workload(args.workload, FP32_FORCE)
+
+def measure(name, steps, fp64_steps, warmup, in_order=False, cpu=False):
+    """All three precision forms of one workload; returns the JSON fields of
+    its line (value, roofline, errors, speed-ups, state update, ...)."""
+    w32, desc = workload(name, FP32_FORCE)
     per_step = 4 * w32.interactions_per_eval()
-
-    t32, kt32, a32, drift32, io32 = timed_run(w32, FP32_FORCE, args.steps, args.warmup, args.in_order)
-    print(f"[bench] {args.workload} FP32-force: {args.steps} steps in {t32:.3f} s", file=sys.stderr, flush=True)
-    k64 = args.fp64_steps if args.fp64_steps is not None else args.steps
-    t64, kt64, a64, drift64, _ = timed_run(w32, FP64, k64, min(args.warmup, 2), args.in_order)
-    print(f"[bench] {args.workload} plain FP64: {k64} steps in {t64:.3f} s", file=sys.stderr, flush=True)
+    t32, kt32, a32, drift32, io32 = timed_run(w32, FP32_FORCE, steps, warmup, in_order)
+    print(f"[bench] {name} FP32-force: {steps} steps in {t32:.3f} s", file=sys.stderr, flush=True)
+    k64 = fp64_steps
+    t64, kt64, a64, drift64, _ = timed_run(w32, FP64, k64, min(warmup, 2), in_order)
+    print(f"[bench] {name} plain FP64: {k64} steps in {t64:.3f} s", file=sys.stderr, flush=True)
     ns = w32.state.n_grav - w32.params.n_planets
     # FP64 with refined hardware reciprocal square roots in every pair law
-    trq, ktrq, arq, driftrq, _ = timed_run(w32, FP64_RSQ, k64, min(args.warmup, 2), args.in_order)
-    print(f"[bench] {args.workload} FP64_RSQ: {k64} steps in {trq:.3f} s", file=sys.stderr, flush=True)
+    trq, ktrq, arq, driftrq, _ = timed_run(w32, FP64_RSQ, k64, min(warmup, 2), in_order)
+    print(f"[bench] {name} FP64_RSQ: {k64} steps in {trq:.3f} s", file=sys.stderr, flush=True)
 
     med32, max32 = force_error(w32, a32)
     med64, max64 = force_error(w32, a64)
 
-    value = per_step * args.steps / t32
+    value = per_step * steps / t32
     v64 = per_step * k64 / t64
     # dominant kernel: massive pairs (swarm workloads) or the grain kernel (debris ring)
     dom = DDYN_KCLASS_PAIRS if w32.state.n_dust == 0 else DDYN_KCLASS_DUST
@@ -385,7 +385,7 @@ def main():
         kname = "k_pairs32"
     elif ns == 0:
         # star + planets only: k_small_steps runs all steps of a call in one launch
-        inter_launch = per_step * args.steps / max(nl, 1)
+        inter_launch = per_step * steps / max(nl, 1)
         kname = "k_small_steps"
     else:
         inter_launch = 4 * w32.state.n_dust * w32.state.n_grav
@@ -405,26 +405,16 @@ def main():
            "kernel_tflops": achrq, "kernel_frac_fp64_peak": achrq / PEAK_FP64_TFLOPS,
            "speedup_fp32_vs_fp64_rsq": value / vrq, "speedup_vs_plain_fp64": vrq / v64, "energy_drift": driftrq,
            "note": "FP64 throughout; 1/R^3 (and the star term's 1/R^2) from v_rsq_f64 + one third-order step"}
-
     out = {
-        "metric": "pairwise interactions/s & RK4 steps/s, FP32-force vs FP64 at 1e-5 rel err",
         "value": value,
-        "unit": "interactions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * t32 / args.steps,
-        "steps_per_s": args.steps / t32,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32 pair forces, fp64 state",
-        "data": "synthetic (seeded Philox)",
+        "steps": steps,
+        "ms_per_step": 1e3 * t32 / steps,
+        "steps_per_s": steps / t32,
         "config": {"workload": f"{w32.name}: {desc}", "n_grav": w32.state.n_grav,
                    "n_dust": w32.state.n_dust, "interactions_per_step": per_step,
                    "parallelism": "single GPU"},
         "force_rel_err": {"median": med32, "max": max32, "vs": "FP64 CPU oracle, sampled targets"},
-        "fp64": {"value": v64, "ms_per_step": 1e3 * t64 / k64, "steps": k64,
+        "fp64": {"value": v64, "ms_per_step": 1e3 * t64 / k64, "steps_per_s": k64 / t64, "steps": k64,
                  "force_rel_err_median": med64, "force_rel_err_max": max64,
                  "kernel_tflops": ach64, "kernel_frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS},
         "speedup_fp32_vs_fp64": value / v64,
@@ -434,18 +424,82 @@ def main():
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(kname, w32.name), "launches": nl, "avg_launch_ms": avg_ms,
-                     "flop_per_launch": FLOP_PER_INTERACTION * inter_launch},
-        "state_update": state_update(w32, kt32, args.steps),
+                     "flop_per_launch": FLOP_PER_INTERACTION * inter_launch,
+                     "note": "achieved = 20 FLOP x interactions per launch / mean HIP-event launch time; "
+                             "traffic = calibrated FETCH_SIZE + WRITE_SIZE bytes per launch from profiles/"},
+        "state_update": state_update(w32, kt32, steps),
         "host_transfer": {"upload_ms": 1e3 * io32[0], "download_ms": 1e3 * io32[1],
-                          "value_incl_transfer": per_step * args.steps / (t32 + io32[0] + io32[1]),
+                          "value_incl_transfer": per_step * steps / (t32 + io32[0] + io32[1]),
                           "note": "one upload and one download of the whole state around the timed steps; "
                                   "value excludes them (state resident in HBM)"},
     }
-    if args.workload == "cfg1":
+    if name == "cfg1":
         out["jacobi_drift"] = jacobi_drift()
-    if not args.no_cpu:
-        cb = cpu_baseline(w32)
-        out["cpu_baseline"] = cb
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(w32)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed RK4 steps (default: the workload's stated count)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="cfg3", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"])
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="steps for the FP64 comparison runs (default: --steps)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-also", action="store_true",
+                    help="skip the swarm and large-disk lines that the debris-ring run adds under 'also'")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
+    ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
+    ap.add_argument("--fp64-rsq", action="store_true",
+                    help="with --adjoint cfg5: FP64 with refined hardware rsqrt in every pair law")
+    ap.add_argument("--in-order", action="store_true",
+                    help="time every pass with one launch per kernel, in order (kernel profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the swarm would shard by target block, the grains by grain range)", file=sys.stderr)
+        return 2
+    if args.adjoint:
+        if args.workload not in ("cfg4", "cfg5"):
+            raise SystemExit("--adjoint runs the cfg4 or cfg5 workload")
+        return adjoint_run(args.workload, FP64 if args.fp64 else FP64_RSQ if args.fp64_rsq else FP32_FORCE)
+
+    steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
+    k64 = args.fp64_steps if args.fp64_steps is not None else steps
+    m = measure(args.workload, steps, k64, args.warmup, args.in_order, cpu=not args.no_cpu)
+    out = {
+        "metric": "pairwise interactions/s & RK4 steps/s, FP32-force vs FP64 at 1e-5 rel err",
+        "value": m.pop("value"),
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": m.pop("ms_per_step"),
+        "steps_per_s": m.pop("steps_per_s"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair forces, fp64 state",
+        "data": "synthetic (seeded Philox)",
+    }
+    m.pop("steps")
+    out.update(m)
+    if args.workload == "cfg3" and not args.no_also and args.steps is None and not args.in_order:
+        # the swarm and the disk at reduced step counts: their own throughput,
+        # roofline and FP32-force speed-up (no CPU baseline)
+        also = {}
+        for name, st32, st64 in (("cfg2", 200, 20), ("cfg5", 2, 1)):
+            a = measure(name, st32, st64, 1)
+            also[name] = {k: a[k] for k in ("value", "ms_per_step", "steps_per_s", "steps", "config",
+                                            "force_rel_err", "speedup_fp32_vs_fp64", "roofline", "state_update")}
+            also[name]["fp64"] = {k: a["fp64"][k] for k in ("value", "ms_per_step", "steps", "kernel_frac_fp64_peak")}
+            also[name]["fp64_rsq"] = {k: a["fp64_rsq"][k] for k in ("value", "ms_per_step", "steps",
+                                                                     "speedup_fp32_vs_fp64_rsq")}
+        out["also"] = also
     print(json.dumps(out), flush=True)
     return 0
 

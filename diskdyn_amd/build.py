@@ -54,23 +54,24 @@ def build_host(verbose=False):
     return HOST_LIB
 
 
-PROBE = os.path.join(HERE, "ddyn_probe")
+COPY_CALIB_SRC = os.path.join(ROOT, "copy_calib.hip")
+COPY_CALIB = os.path.join(ROOT, "copy_calib")
 
 
-def build_probe(verbose=False):
-    """Rate microbenchmarks and the counter-calibration copy kernel (measurement)."""
-    tmp = PROBE + ".tmp"
+def build_copy_calib(verbose=False):
+    """The copy kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
+    (measurement only, not part of the library)."""
+    tmp = COPY_CALIB + ".tmp"
     flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
-    _run([HIPCC] + flags + [os.path.join(CSRC, "ddyn_probe.hip"), "-o", tmp], "hipcc probe")
-    os.replace(tmp, PROBE)
+    _run([HIPCC] + flags + [COPY_CALIB_SRC, "-o", tmp], "hipcc copy_calib")
+    os.replace(tmp, COPY_CALIB)
     if verbose:
-        print("built", PROBE)
+        print("built", COPY_CALIB)
 
 
 def build_all(verbose=False):
     build_hip(verbose)
     build_host(verbose)
-    build_probe(verbose)
 
 
 if __name__ == "__main__":

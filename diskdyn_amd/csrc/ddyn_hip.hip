@@ -556,7 +556,7 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
         double m = 1.0, bet = 0.0, betsw = 0.0, qe = 0.0;
         if (valid) {
             m = A.mu[t];
-            const int32_t id = A.id[t];
+            const int32_t id = A.id ? A.id[t] : -1;
             if (A.beta_p) {
                 bet = A.beta_p[g];
                 betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
@@ -664,7 +664,7 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
                 A.ox[g] = gone ? 0.0 : tot[0];
                 A.oy[g] = gone ? 0.0 : tot[1];
                 A.oz[g] = gone ? 0.0 : tot[2];
-                A.removed[g] = gone ? 1 : 0;
+                if (A.removed) A.removed[g] = gone ? 1 : 0;
             } else if (s < 3) {
                 // next stage (src/gpu_functions.cu:737-743, 800-806, 863-869)
 #pragma unroll
@@ -1487,6 +1487,13 @@ __global__ __launch_bounds__(kWG) void k_potential(const double4* __restrict__ p
     if (i < n_grav) phi[i] = u;
 }
 
+// The library's FP64 reciprocal square root on n inputs (accuracy check).
+__global__ void k_rsqrt_fp64(const double* __restrict__ x, double* __restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (i < n) y[i] = rsqrt_fp64(x[i]);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1659,27 +1666,47 @@ void destroy_events(ddyn_ctx* c)
 
 int n_swarm(const ddyn_ctx* c) { return (int)(c->n_grav - c->prm.n_planets); }
 
-// Choose targets per lane and the source-slice split so that the pair kernel
-// launches about kTargetWGs workgroups.
-void plan_pairs(ddyn_ctx* c)
+// The source-slice split of the pair kernels for n_grav targets and ns swarm
+// sources, so that a launch has about kTargetWGs workgroups; at most
+// max_nslice slices (when > 0).
+void pair_split(int64_t n_grav, int ns, int max_nslice, int* nslice, int* slice_len)
 {
-    const int ns = n_swarm(c);
-    c->tpt = 1;
-    if (ns <= 0) { c->nslice = 0; c->slice_len = 0; return; }
-    const int tblocks = (int)((c->n_grav + (int64_t)kWG * c->tpt - 1) / ((int64_t)kWG * c->tpt));
+    if (ns <= 0) { *nslice = 0; *slice_len = 0; return; }
+    const int tblocks = (int)((n_grav + kWG - 1) / kWG);
     int want = (kTargetWGs + tblocks - 1) / tblocks;
     // few targets (a small swarm): slices down to one tile, the launch is latency-bound
     const int min_slice = (tblocks < 64) ? kTile : kMinSlice;
     const int max_slices = std::max(1, ns / min_slice);
     if (want > max_slices) want = max_slices;
+    if (max_nslice > 0 && want > max_nslice) want = max_nslice;
     if (want < 1) want = 1;
     int len = (ns + want - 1) / want;
     len = ((len + kTile - 1) / kTile) * kTile;
-    c->slice_len = len;
-    c->nslice = (ns + len - 1) / len;
+    *slice_len = len;
+    *nslice = (ns + len - 1) / len;
 }
 
-int check_ready(ddyn_ctx* c)
+// Choose the source-slice split of the uploaded state (one target per lane).
+void plan_pairs(ddyn_ctx* c)
+{
+    c->tpt = 1;
+    pair_split(c->n_grav, n_swarm(c), 0, &c->nslice, &c->slice_len);
+}
+
+// Without softening the default FP64 radius 10 eps is 0: no swarm pair would
+// be redone in FP64, unsoftened close pairs would come from FP32 positions and
+// an exact hit on a swarm body would never be seen.  So a swarm in the
+// FP32-force form with eps = 0 needs an FP64 radius of its own.
+int check_fp64_radius(ddyn_ctx* c, int64_t n_swarm_bodies)
+{
+    if (c->prm.precision == DDYN_FP32_FORCE && n_swarm_bodies > 0 && c->prm.eps == 0.0 &&
+        !(c->prm.fp64_radius > 0.0))
+        return fail(c, DDYN_ERR_ARG, "DDYN_FP32_FORCE with a swarm and eps = 0 needs fp64_radius > 0");
+    return DDYN_OK;
+}
+
+// Parameters and a state are present (enough for diagnostics).
+int check_state(ddyn_ctx* c)
 {
     if (!c) return DDYN_ERR_ARG;
     if (!c->have_params) return fail(c, DDYN_ERR_STATE, "ddyn_set_params has not been called");
@@ -1689,6 +1716,13 @@ int check_ready(ddyn_ctx* c)
     return DDYN_OK;
 }
 
+// ... and the forces can be evaluated (steps, accelerations, the adjoint).
+int check_ready(ddyn_ctx* c)
+{
+    const int rc = check_state(c);
+    return rc ? rc : check_fp64_radius(c, n_swarm(c));
+}
+
 double eps2_of(const ddyn_ctx* c) { return c->prm.eps * c->prm.eps; }
 
 double r64_of(const ddyn_ctx* c)
@@ -1696,27 +1730,32 @@ double r64_of(const ddyn_ctx* c)
     return c->prm.fp64_radius > 0.0 ? c->prm.fp64_radius : 10.0 * c->prm.eps;
 }
 
-// Swarm-pair forces at stage s into c->part.
-int launch_pairs(ddyn_ctx* c, int s)
+// Swarm-pair forces of the stage positions p32/p64 (ng bodies) into part, in
+// nslice slices of slice_len sources.
+int launch_pairs_on(ddyn_ctx* c, const float4* p32, const double4* p64, int ng, int nslice, int slice_len,
+                    double* part)
 {
-    if (c->nslice == 0) return DDYN_OK;
+    if (nslice == 0) return DDYN_OK;
     TimedLaunch tl(c, DDYN_KCLASS_PAIRS);
-    const int ng = (int)c->n_grav, np = c->prm.n_planets;
-    dim3 grid((unsigned)((ng + kWG * c->tpt - 1) / (kWG * c->tpt)), (unsigned)c->nslice);
+    const int np = c->prm.n_planets;
+    dim3 grid((unsigned)((ng + kWG - 1) / kWG), (unsigned)nslice);
     if (c->prm.precision == DDYN_FP32_FORCE) {
         const double r64 = r64_of(c);
-        k_pairs32<1><<<grid, kWG, 0, c->stream>>>(c->p32[s], c->p64[s], ng, np, c->slice_len,
-                                                   (float)eps2_of(c), (float)(r64 * r64),
-                                                   eps2_of(c), c->part);
+        k_pairs32<1><<<grid, kWG, 0, c->stream>>>(p32, p64, ng, np, slice_len, (float)eps2_of(c),
+                                                   (float)(r64 * r64), eps2_of(c), part);
     } else if (c->prm.precision == DDYN_FP64_RSQ) {
-        k_pairs64<1, true><<<grid, kWG, 0, c->stream>>>(c->p64[s], ng, np, c->slice_len, eps2_of(c),
-                                                         c->part);
+        k_pairs64<1, true><<<grid, kWG, 0, c->stream>>>(p64, ng, np, slice_len, eps2_of(c), part);
     } else {
-        k_pairs64<1, false><<<grid, kWG, 0, c->stream>>>(c->p64[s], ng, np, c->slice_len, eps2_of(c),
-                                                          c->part);
+        k_pairs64<1, false><<<grid, kWG, 0, c->stream>>>(p64, ng, np, slice_len, eps2_of(c), part);
     }
     DDYN_HIP(c, hipGetLastError());
     return DDYN_OK;
+}
+
+// Swarm-pair forces at stage s into c->part.
+int launch_pairs(ddyn_ctx* c, int s)
+{
+    return launch_pairs_on(c, c->p32[s], c->p64[s], (int)c->n_grav, c->nslice, c->slice_len, c->part);
 }
 
 StageArgs stage_args(ddyn_ctx* c, double h)
@@ -1738,7 +1777,7 @@ StageArgs stage_args(ddyn_ctx* c, double h)
 int launch_stage(ddyn_ctx* c, int s, const StageArgs& A)
 {
     TimedLaunch tl(c, DDYN_KCLASS_STAGE);
-    const unsigned blocks = (unsigned)((c->n_grav + kWG - 1) / kWG);
+    const unsigned blocks = (unsigned)((A.n_grav + kWG - 1) / kWG);
     if (c->prm.precision == DDYN_FP64_RSQ) {
         switch (s) {
             case 0: k_stage<0, true><<<blocks, kWG, 0, c->stream>>>(A); break;
@@ -1785,10 +1824,10 @@ DustArgs dust_args(ddyn_ctx* c, double dt)
 
 int launch_dust(ddyn_ctx* c, const DustArgs& A, bool accel, hipStream_t st = nullptr)
 {
-    if (c->n_dust == 0) return DDYN_OK;
+    if (A.n_dust == 0) return DDYN_OK;
     if (!st) st = c->stream;
     TimedLaunch tl(c, DDYN_KCLASS_DUST, st);
-    const unsigned blocks = (unsigned)((c->n_dust + 63) / 64);
+    const unsigned blocks = (unsigned)((A.n_dust + 63) / 64);
     const int form = c->prm.precision;
     if (accel) {
         if (form == DDYN_FP32_FORCE) k_dust<true, true><<<blocks, kDustWG, 0, st>>>(A);
@@ -2357,7 +2396,7 @@ int ddyn_adjoint_reverse(ddyn_ctx* c, const double* lam_r, const double* lam_v, 
 
 int ddyn_energy(ddyn_ctx* c, double* out7)
 {
-    int rc = check_ready(c);
+    int rc = check_state(c);
     if (rc) return rc;
     if (!out7) return fail(c, DDYN_ERR_ARG, "out7 must not be NULL");
     DDYN_HIP(c, hipSetDevice(c->device));
@@ -2410,10 +2449,27 @@ int ddyn_set_timing(ddyn_ctx* c, int enable)
 
 int ddyn_pair_plan(ddyn_ctx* c, int32_t* nslice, int32_t* slice_len)
 {
-    const int rc = check_ready(c);
+    const int rc = check_state(c);
     if (rc) return rc;
     if (nslice) *nslice = c->nslice;
     if (slice_len) *slice_len = c->slice_len;
+    return DDYN_OK;
+}
+
+int ddyn_eval_rsqrt_fp64(ddyn_ctx* c, int64_t n, const double* x, double* y)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (n < 0 || n > ((int64_t)1 << 30) || (n > 0 && (!x || !y)))
+        return fail(c, DDYN_ERR_ARG, "need 0 <= n <= 2^30 and non-NULL arrays");
+    if (n == 0) return DDYN_OK;
+    DDYN_HIP(c, hipSetDevice(c->device));
+    DevBuf dx, dy;
+    if (dalloc(c, dx.p, (size_t)n) || dalloc(c, dy.p, (size_t)n)) return DDYN_ERR_NOMEM;
+    DDYN_HIP(c, hipMemcpyAsync(dx.p, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_rsqrt_fp64<<<(unsigned)((n + kWG - 1) / kWG), kWG, 0, c->stream>>>(dx.p, dy.p, n);
+    DDYN_HIP(c, hipGetLastError());
+    DDYN_HIP(c, hipMemcpyAsync(y, dy.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DDYN_HIP(c, hipStreamSynchronize(c->stream));
     return DDYN_OK;
 }
 

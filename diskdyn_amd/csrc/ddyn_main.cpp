@@ -12,7 +12,9 @@
 //
 // Options: --fp32 (FP32-force form), --fp64-rsq (FP64 with refined hardware
 //          reciprocal square roots in every pair law), --eps <AU> (softening of
-//          swarm pairs), --rmin <AU> (sublimation radius), --device <n>, --quiet.
+//          swarm pairs), --r64 <AU> (FP32-force form: swarm pairs closer than
+//          this are evaluated in FP64; default 10 eps, or 0.01 AU without
+//          softening), --rmin <AU> (sublimation radius), --device <n>, --quiet.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -32,7 +34,7 @@ constexpr double kYear = 3.1556926e7;
 void usage()
 {
     std::fprintf(stderr,
-                 "Usage: ddyn -i <init.param> | -m <model.dat> [--fp32 | --fp64-rsq] [--eps AU] [--rmin AU] "
+                 "Usage: ddyn -i <init.param> | -m <model.dat> [--fp32 | --fp64-rsq] [--eps AU] [--r64 AU] [--rmin AU] "
                  "[--device n] [--quiet]\n");
 }
 
@@ -60,7 +62,7 @@ int main(int argc, char** argv)
     const char* param = nullptr;
     const char* model = nullptr;
     int precision = DDYN_FP64, device = 0, quiet = 0;
-    double eps = 0.0, rmin = 0.0;
+    double eps = 0.0, rmin = 0.0, r64 = 0.0;
     for (int a = 1; a < argc; ++a) {
         const std::string s = argv[a];
         if (s == "-i" && a + 1 < argc) param = argv[++a];
@@ -69,6 +71,7 @@ int main(int argc, char** argv)
         else if (s == "--fp64-rsq") precision = DDYN_FP64_RSQ;
         else if (s == "--eps" && a + 1 < argc) eps = std::atof(argv[++a]);
         else if (s == "--rmin" && a + 1 < argc) rmin = std::atof(argv[++a]);
+        else if (s == "--r64" && a + 1 < argc) r64 = std::atof(argv[++a]);
         else if (s == "--device" && a + 1 < argc) device = std::atoi(argv[++a]);
         else if (s == "--quiet") quiet = 1;
         else { usage(); return s == "-h" ? 0 : 1; }
@@ -120,6 +123,18 @@ int main(int argc, char** argv)
         }
         st.n_dust = run.n_dust;
         st.n_grav = run.n_all - run.n_dust;
+        // The data file carries beta per grain only; the reference recomputes
+        // beta_sw and the charge from the optical data on restart
+        // (src/grid.cu:296).  Without those tables the wind drag and the
+        // Lorentz force would silently be zero, so such a run is refused.
+        if (st.n_dust > 0 && (cfg.SWQ == 1 || cfg.FLQ == 1)) {
+            std::fprintf(stderr, "ddyn: %s has stellar-wind drag or the Lorentz force on, which need "
+                                 "beta_sw and charge tables that a data file does not carry\n", model);
+            return 2;
+        }
+        if (st.n_dust > 0 && rmin <= 0.0)
+            std::fprintf(stderr, "ddyn: warning: no sublimation radius (--rmin): grains are never removed "
+                                 "near the star\n");
     }
     run.Rmax = 0.0;
     for (int k = 0; k < cfg.n_disks; ++k) run.Rmax = std::max(run.Rmax, cfg.disks[k].Rout);
@@ -144,7 +159,12 @@ int main(int argc, char** argv)
     prm.Bstar = cfg.Bstar;
     prm.Rstar = cfg.rstar;
     prm.light = 0.0;
-    prm.fp64_radius = 0.0;
+    prm.fp64_radius = r64;
+    if (precision == DDYN_FP32_FORCE && eps == 0.0 && !(r64 > 0.0)) {
+        // unsoftened pairs: an FP64 radius of its own (10 eps would be 0)
+        prm.fp64_radius = 0.01;
+        if (!quiet) std::printf("ddyn: FP32-force form without softening: pairs within 0.01 AU in FP64\n");
+    }
     if (check(ctx, ddyn_set_params(ctx, &prm), "set_params") ||
         check(ctx, ddyn_upload_state(ctx, st.n_grav, st.n_dust, st.x.data(), st.y.data(), st.z.data(), st.vx.data(),
                                      st.vy.data(), st.vz.data(), st.mu.data(), st.N.data(), st.id.data()),

@@ -52,6 +52,7 @@ SIGNATURES = {
     "ddyn_set_timing": (ctypes.c_int, [_VP, ctypes.c_int]),
     "ddyn_read_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _I64, _D]),
     "ddyn_pair_plan": (ctypes.c_int, [_VP, _I32, _I32]),
+    "ddyn_eval_rsqrt_fp64": (ctypes.c_int, [_VP, ctypes.c_int64, _D, _D]),
 }
 
 _lib = None
@@ -238,6 +239,14 @@ class Engine:
         ln = ctypes.c_int32(0)
         self._check(self.lib.ddyn_pair_plan(self.ctx, ctypes.byref(ns), ctypes.byref(ln)), "ddyn_pair_plan")
         return ns.value, ln.value
+
+
+    def rsqrt_fp64(self, x):
+        """The library's device FP64 reciprocal square root of every x > 0."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._check(self.lib.ddyn_eval_rsqrt_fp64(self.ctx, len(x), _dp(x), _dp(y)), "ddyn_eval_rsqrt_fp64")
+        return y
 
 
 def run(workload, device=0, nsteps=None, precision=None):

@@ -68,7 +68,10 @@ typedef struct {
     double  Rstar;       /* stellar radius [AU] (cfg->rstar) */
     double  light;       /* speed of light [AU/yr]; 0 selects lightAU = 63239.6717367 */
     double  fp64_radius; /* DDYN_FP32_FORCE: swarm pairs closer than this [AU] are evaluated in
-                            FP64; <= 0 selects 10*eps */
+                            FP64; <= 0 selects 10*eps.  With a swarm and eps = 0 (the
+                            reference law) it must be > 0, else ddyn_step, ddyn_accel and
+                            the adjoint return DDYN_ERR_ARG: the FP64 redo is what evaluates
+                            close pairs and detects exact hits */
     int32_t schedule;    /* 0: fuse many steps per launch where the system allows it (star +
                             planets only, <= 32 massive bodies), and with grains run the grain
                             kernel of step k beside the massive stages of step k+1 on a second
@@ -183,6 +186,12 @@ int ddyn_read_timing(ddyn_ctx *ctx, int32_t kclass, int64_t *launches, double *t
  * slice adds 3 FP64 partials per massive body per stage.  Needs parameters
  * and a state.  No reference counterpart (measurement support). */
 int ddyn_pair_plan(ddyn_ctx *ctx, int32_t *nslice, int32_t *slice_len);
+
+/* y[i] = the library's FP64 reciprocal square root of x[i] > 0, evaluated on the
+ * device (the v_rsq_f64 estimate refined by one third-order step, used by every
+ * pair law of DDYN_FP64_RSQ and by the adjoint's Jacobian products).  n <= 2^30;
+ * host arrays; synchronises.  No reference counterpart (accuracy check). */
+int ddyn_eval_rsqrt_fp64(ddyn_ctx *ctx, int64_t n, const double *x, double *y);
 
 #ifdef __cplusplus
 }

@@ -166,3 +166,35 @@ def test_reference_example_params_parse():
         mu = np.array(cfg.pm[:n])
         P = np.array([[cfg.pvx[j], cfg.pvy[j], cfg.pvz[j]] for j in range(n)])
         assert np.abs((mu[:, None] * P).sum(0)).max() <= 1e-12 * max(1.0, np.abs(mu[:, None] * P).max())
+
+
+def test_restart_with_wind_drag_and_grains_is_refused(tmp_path):
+    """A data file carries beta per grain but no beta_sw or charge tables, so
+    ddyn -m refuses a restart with grains and wind drag or the Lorentz force
+    on, instead of running them at zero (the refusal comes before any GPU
+    call)."""
+    import subprocess
+
+    from diskdyn_amd import hostio
+    from diskdyn_amd.system import make_state
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    ddyn = os.path.join(root, "diskdyn_amd", "ddyn")
+    p = tmp_path / "sys.param"
+    p.write_text(PARAM)
+    cfg = hostio.read_param(p)
+    for swq, flq in ((1, 0), (0, 1)):
+        cfg.SWQ, cfg.FLQ = swq, flq
+        st = make_state(cfg.n_planets, 3)
+        for j in range(cfg.n_planets):
+            st.x[j], st.y[j], st.vy[j], st.mu[j] = cfg.px[j], cfg.py[j], cfg.pvy[j], cfg.pm[j]
+        st.x[cfg.n_planets:] = [30.0, 31.0, 32.0]
+        st.vy[cfg.n_planets:] = 1.1
+        st.beta_p = np.full(3, 0.1)
+        run = hostio.RunInfo()
+        run.n_mass, run.n_dust, run.n_all, run.Rmax = 0, 3, st.n_all, 1e300
+        f = tmp_path / f"grains_{swq}{flq}.dat"
+        hostio.write_datafile(f, cfg, st, run)
+        r = subprocess.run([ddyn, "-m", str(f), "--quiet"], cwd=tmp_path, capture_output=True, text=True,
+                           timeout=60)
+        print(r.returncode, r.stderr)
+        assert r.returncode == 2 and "beta_sw and charge tables" in r.stderr
