@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -1497,6 +1498,17 @@ __global__ void k_rsqrt_fp64(const double* __restrict__ x, double* __restrict__ 
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Static scratch of ddyn_accel_noalloc: the stage-0 positions (FP64 and FP32
+// shadow) and the pair kernels' slice partials live here instead of in
+// library allocations.  32 MiB per device for the life of the process; every
+// call on a device shares it, so calls that overlap on one device race.
+// Host code never passes `scratch` itself to a kernel: in host code the name
+// is a stand-in, not a device address.  ddyn_create looks up each device's
+// copy once with hipGetSymbolAddress and every use goes through that address.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+// ---------------------------------------------------------------------------
 // Context and C ABI
 // ---------------------------------------------------------------------------
 struct ddyn_ctx {
@@ -1548,6 +1560,27 @@ struct ddyn_ctx {
 };
 
 namespace {
+
+// Device address of `scratch` on each device, looked up once per device.
+constexpr int kMaxDevices = 64;
+void* g_scratch_addr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// With `device` current: the device's scratch address, looked up on first use.
+hipError_t scratch_addr(int device, void** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
 
 int fail(ddyn_ctx* c, int code, const std::string& msg)
 {
@@ -1917,6 +1950,10 @@ int ddyn_create(ddyn_ctx** out, int device)
         ddyn_destroy(c);
         return DDYN_ERR_NOMEM;
     }
+    // the device's static scratch of ddyn_accel_noalloc (looked up once per device)
+    void* sp = nullptr;
+    const hipError_t se = scratch_addr(device, &sp);
+    if (se != hipSuccess) c->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
     *out = c;
     return DDYN_OK;
 }
@@ -2135,6 +2172,56 @@ int ddyn_accel(ddyn_ctx* c, double* ax, double* ay, double* az, int32_t* removed
     if (removed && c->n_dust)
         DDYN_HIP(c, hipMemcpy(removed, c->rem, (size_t)c->n_dust * sizeof(int32_t), hipMemcpyDeviceToHost));
     return DDYN_OK;
+}
+
+int ddyn_accel_noalloc(ddyn_ctx* c, int64_t n_grav, int64_t n_dust, const double* x, const double* y,
+                       const double* z, const double* vx, const double* vy, const double* vz, const double* mu,
+                       const int32_t* id, const double* beta_p, const double* betasw_p, double* ax, double* ay,
+                       double* az, int32_t* removed)
+{
+    if (!c) return DDYN_ERR_ARG;
+    if (!c->have_params) return fail(c, DDYN_ERR_STATE, "ddyn_set_params has not been called");
+    if (n_grav < 1 || n_dust < 0 || n_grav > (int64_t)1 << 30 || n_dust > (int64_t)1 << 30)
+        return fail(c, DDYN_ERR_ARG, "need 1 <= n_grav and 0 <= n_dust, each below 2^30");
+    if (!x || !y || !z || !vx || !vy || !vz || !mu || !ax || !ay || !az)
+        return fail(c, DDYN_ERR_ARG, "state and output arrays must not be NULL");
+    if (c->prm.n_planets > n_grav) return fail(c, DDYN_ERR_ARG, "n_planets exceeds n_grav");
+    int rc = check_fp64_radius(c, n_grav - c->prm.n_planets);
+    if (rc) return rc;
+    // scratch layout: stage-0 positions (double4, float4) then the slice partials
+    const size_t ng = (size_t)n_grav;
+    const size_t pos_bytes = ((ng * (sizeof(double4) + sizeof(float4)) + 255) / 256) * 256;
+    if (pos_bytes + 3 * ng * sizeof(double) > kScratchBytes)
+        return fail(c, DDYN_ERR_ARG, "n_grav too large for the static scratch of ddyn_accel_noalloc (" +
+                                         std::to_string(kScratchBytes >> 20) + " MiB: at most " +
+                                         std::to_string((kScratchBytes - 256) / 72) + " massive bodies)");
+    const int max_nslice = (int)std::min<size_t>((kScratchBytes - pos_bytes) / (3 * ng * sizeof(double)), 1 << 20);
+    int nslice = 0, slice_len = 0;
+    pair_split(n_grav, (int)(n_grav - c->prm.n_planets), max_nslice, &nslice, &slice_len);
+    DDYN_HIP(c, hipSetDevice(c->device));
+    void* base = nullptr;
+    DDYN_HIP(c, scratch_addr(c->device, &base));
+    double4* p64 = (double4*)base;
+    float4* p32 = (float4*)(p64 + ng);
+    double* part = (double*)((char*)base + pos_bytes);
+    k_init_pos<<<(unsigned)((ng + kWG - 1) / kWG), kWG, 0, c->stream>>>((int)ng, x, y, z, mu, p64, p32);
+    DDYN_HIP(c, hipGetLastError());
+    if ((rc = launch_pairs_on(c, p32, p64, (int)ng, nslice, slice_len, part))) return rc;
+    StageArgs S{};
+    S.n_grav = (int)ng; S.n_planets = c->prm.n_planets; S.nslice = nslice; S.part = part;
+    S.p64[0] = p64; S.p32[0] = p32;
+    S.ox = ax; S.oy = ay; S.oz = az;
+    if ((rc = launch_stage(c, 4, S))) return rc;
+    DustArgs D = dust_args(c, 0.0);
+    D.n_grav = (int)ng; D.n_dust = (int)n_dust;
+    for (int s = 0; s < 4; ++s) { D.p64[s] = p64; D.p32[s] = p32; }
+    D.x = const_cast<double*>(x); D.y = const_cast<double*>(y); D.z = const_cast<double*>(z);
+    D.vx = const_cast<double*>(vx); D.vy = const_cast<double*>(vy); D.vz = const_cast<double*>(vz);
+    D.mu = mu; D.N = nullptr; D.id = id;
+    D.beta_p = beta_p; D.betasw_p = beta_p ? betasw_p : nullptr;
+    D.ox = ax + ng; D.oy = ay + ng; D.oz = az + ng;
+    D.removed = removed;
+    return launch_dust(c, D, true);
 }
 
 int ddyn_download_state(ddyn_ctx* c, double* x, double* y, double* z, double* vx, double* vy,

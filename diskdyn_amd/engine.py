@@ -43,6 +43,7 @@ SIGNATURES = {
     "ddyn_upload_grain_beta": (ctypes.c_int, [_VP, _D, _D]),
     "ddyn_step": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int64]),
     "ddyn_accel": (ctypes.c_int, [_VP, _D, _D, _D, _I32]),
+    "ddyn_accel_noalloc": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_int64] + [_VP] * 14),
     "ddyn_download_state": (ctypes.c_int, [_VP] + [_D] * 7),
     "ddyn_energy": (ctypes.c_int, [_VP, _D]),
     "ddyn_adjoint": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int64, ctypes.c_int64] + [_D] * 6),
@@ -174,6 +175,12 @@ class Engine:
         rem = np.zeros(max(self.n_dust, 1), dtype=np.int32)
         self._check(self.lib.ddyn_accel(self.ctx, _dp(ax), _dp(ay), _dp(az), _ip(rem)), "ddyn_accel")
         return np.stack([ax, ay, az], axis=1), rem[: self.n_dust]
+
+    def accel_noalloc(self, n_grav, n_dust, x, y, z, vx, vy, vz, mu, id_, beta_p, betasw_p, ax, ay, az, removed):
+        """ddyn_accel_noalloc on device addresses (ints, or None for the
+        optional arrays); asynchronous on the context's stream."""
+        self._check(self.lib.ddyn_accel_noalloc(self.ctx, int(n_grav), int(n_dust), x, y, z, vx, vy, vz, mu, id_,
+                                                beta_p, betasw_p, ax, ay, az, removed), "ddyn_accel_noalloc")
 
     def download(self, st: DiskState = None):
         n = self.n_grav + self.n_dust

@@ -122,6 +122,27 @@ int ddyn_step(ddyn_ctx *ctx, double dt, int64_t nsteps);
  * No reference counterpart (the reference never exposes accelerations). */
 int ddyn_accel(ddyn_ctx *ctx, double *ax, double *ay, double *az, int32_t *removed);
 
+/* ddyn_accel without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No reference counterpart.  The inputs and outputs are the
+ * caller's DEVICE arrays, not the context's state: x..vz, mu (n_grav + n_dust
+ * each, reference order), id (n_grav + n_dust, may be NULL: every grain without
+ * a bin), beta_p and betasw_p (n_dust, may be NULL: the context's bin tables),
+ * ax, ay, az (n_grav + n_dust) and removed (n_dust, may be NULL).  Uses the
+ * context's parameters, bin tables and main stream; the stage-0 positions and
+ * the pair kernels' slice partials live in a static 32 MiB device array of the
+ * library (per device, held until the process ends), which limits n_grav to
+ * about 466,000 (DDYN_ERR_ARG beyond).  Asynchronous: no allocation, no copy
+ * and no synchronisation; call ddyn_synchronize (or wait on the stream) before
+ * reading the outputs.  Every call on a device shares that device's array, so
+ * calls that overlap on one device (two contexts, two host threads) race.
+ * Results equal ddyn_accel's for the same state up to the order of the slice
+ * sums (bit for bit when the pair split is the same). */
+int ddyn_accel_noalloc(ddyn_ctx *ctx, int64_t n_grav, int64_t n_dust, const double *x, const double *y,
+                       const double *z, const double *vx, const double *vy, const double *vz,
+                       const double *mu, const int32_t *id, const double *beta_p, const double *betasw_p,
+                       double *ax, double *ay, double *az, int32_t *removed);
+
 /* Download the state (replaces sync_from_gpu, src/init.cu:670-682).  Unlike
  * the reference, N is downloaded too so removals are visible.  Any pointer may
  * be NULL to skip that array. */
