@@ -4,7 +4,8 @@
   -O3, C++17, -ffp-contract=off (every fused multiply-add in the kernels is
   written out explicitly, so rounding is fixed) and the packed-FP32 target
   feature switched off.
-* libddyn_host.so .param reader, time-step rule, data-file writer/reader
+* libddyn_host.so .param reader, time-step rule, data-file writer/reader,
+  optical data, dust grid, grain tables and particle generation
   (include/ddyn_host.h), host C++ only.
 * ddyn            command-line driver linking both.
 """
@@ -43,7 +44,8 @@ def build_hip(verbose=False):
 
 def build_host(verbose=False):
     tmp = HOST_LIB + ".tmp"
-    _run([CXX] + HOST_FLAGS + ["-shared", os.path.join(CSRC, "ddyn_host.cpp"), "-o", tmp], "host library")
+    _run([CXX] + HOST_FLAGS + ["-shared", os.path.join(CSRC, "ddyn_host.cpp"), os.path.join(CSRC, "ddyn_setup.cpp"),
+                               "-o", tmp], "host library")
     os.replace(tmp, HOST_LIB)
     tmp = CLI + ".tmp"
     _run([CXX] + HOST_FLAGS + [os.path.join(CSRC, "ddyn_main.cpp"), "-o", tmp, "-L" + HERE,

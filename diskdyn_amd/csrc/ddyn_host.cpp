@@ -1,6 +1,7 @@
 // ddyn_host.cpp -- .param reader, time-step rule and data-file writer/reader
 // of the dynamical hot path (see include/ddyn_host.h).  Host C++ only.
 #include "ddyn_host.h"
+#include "ddyn_host_impl.h"
 
 #include <cctype>
 #include <cmath>
@@ -62,6 +63,10 @@ const char* error_text(int code)
     }
 }
 
+}  // namespace
+
+namespace ddyn_impl {
+
 // Whitespace tokenizer with double-quoted tokens (quotes removed), as the
 // reference's tokenize_spaces/remove_quotes do; at most `maxtok` tokens.
 std::vector<std::string> tokenize(const char* line, size_t maxtok)
@@ -83,6 +88,13 @@ std::vector<std::string> tokenize(const char* line, size_t maxtok)
     return out;
 }
 
+}  // namespace ddyn_impl
+
+namespace {
+
+using ddyn_impl::coe2rv;
+using ddyn_impl::tokenize;
+
 double num(const std::string& s) { return std::strtod(s.c_str(), nullptr); }
 long lnum(const std::string& s) { return std::strtol(s.c_str(), nullptr, 10); }
 
@@ -97,6 +109,10 @@ void copy_str(char* dst, const std::string& s)
 {
     std::snprintf(dst, DDYN_STR, "%s", s.c_str());
 }
+
+}  // namespace
+
+namespace ddyn_impl {
 
 // coe2rv (src/functions.cu:6-46), mu = (M + m)(1 - beta).
 void coe2rv(double M, double m, double a, double e, double i, double w, double o, double nu,
@@ -116,6 +132,10 @@ void coe2rv(double M, double m, double a, double e, double i, double w, double o
     r[0] = xe * r11 + ye * r12; r[1] = xe * r21 + ye * r22; r[2] = xe * r31 + ye * r32;
     r[3] = vxe * r11 + vye * r12; r[4] = vxe * r21 + vye * r22; r[5] = vxe * r31 + vye * r32;
 }
+
+}  // namespace ddyn_impl
+
+namespace {
 
 // recalc_COM (src/read_in.cu:344-375).
 void recalc_com(ddyn_config* c)

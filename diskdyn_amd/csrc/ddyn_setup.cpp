@@ -1,0 +1,650 @@
+// ddyn_setup.cpp -- what a .param run needs before its first step: the optical
+// data of the grains and the star, the dust size grid, the per-bin beta,
+// beta_sw and charge tables with the sublimation radius, and the particles of
+// the disks and blobs (see include/ddyn_host.h).  Host C++ only.
+//
+// The reference does this in main and Grid::setup (src/ddyn.cu:78-96,
+// src/grid.cu:62-189) with add_optical_data and add_specfile
+// (src/read_in.cu:377-506), detminmax, confCdisks and confCblobs
+// (src/read_in.cu:236-342), interpolate_dust_to_grid,
+// interpolate_stellar_to_grid and calculate_beta (src/functions.cu:243-388),
+// calculate_dust_temp (src/grid.cu:754-792) and getbod, genorb_disk and
+// genorb_blob (src/functions.cu:157-240).  The file formats, units and the
+// formulas are the reference's; the random numbers are not (see Rng).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "ddyn_host.h"
+#include "ddyn_host_impl.h"
+
+struct ddyn_optics {
+    int n_wav = 0, n_theta = 0, n_sizes = 0;
+    double dmin = 0.0, dmax = 0.0;           // optical size range [m] (the table header)
+    std::vector<double> w;                   // wavelengths [m], n_wav
+    std::vector<double> s;                   // grain sizes [m], n_sizes
+    std::vector<double> Qabs, Qsca;          // [size][wav]
+    std::vector<double> Qpf;                 // phase function [size][theta][wav]
+    std::vector<double> spec_w, spec_F;      // stellar spectrum: wavelength [m], flux at the observer
+};
+
+namespace {
+
+// Constants of src/ddyn.cuh:16-38.
+constexpr double kAU = 1.49598e11;
+constexpr double kMSun = 1.98892e30;
+constexpr double kYear = 3.1556926e7;
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kPI2 = 6.28318530717958623200;
+constexpr double kGrav = 4.0 * kPI * kPI;
+constexpr double kC = 299792458.0;
+constexpr double kBoltz = 1.3806503E-23;
+constexpr double kPlanck = 6.626068E-34;
+constexpr double kTsub = 2000.0;         // sublimation temperature [K]
+constexpr double kMgrid = 1.1;           // mass ratio of neighbouring dust bins
+constexpr double kCd = 2.0;              // drag coefficient factor
+constexpr double kLsunW = 3.827e26;      // solar luminosity [W] (src/functions.cu:355)
+constexpr int kMaxBins = 2700;           // size bins the device tables hold (nsize)
+
+// Error numbers of the reference's message table (src/error.cu:5-64).
+constexpr int kErrFile = 39;       // a file could not be opened
+constexpr int kErrOptics = 42;     // optical data not well formed
+constexpr int kErrInterp = 54;     // interpolation / grid error
+
+bool read_lines(const char* path, std::vector<std::string>& lines)
+{
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::string cur;
+    int ch;
+    while ((ch = std::fgetc(f)) != EOF) {
+        if (ch == '\n') { lines.push_back(cur); cur.clear(); }
+        else cur.push_back((char)ch);
+    }
+    if (!cur.empty()) lines.push_back(cur);
+    std::fclose(f);
+    return true;
+}
+
+bool to_double(const std::string& t, double& v)
+{
+    char* end = nullptr;
+    v = std::strtod(t.c_str(), &end);
+    return end != t.c_str();
+}
+
+// The composition table (src/read_in.cu:419-506): a first line that is
+// skipped; a definition line whose tokens 1-5 are n_wav, n_theta, the number
+// of sizes and the size range [um]; then per size a line "# <size [um]>",
+// n_wav lines "<wavelength [um]> <Qabs> <Qsca>" (wavelengths taken from the
+// first size) and the phase function, n_theta x n_wav values ten per line,
+// theta-major.
+int read_composition(const char* path, ddyn_optics& o)
+{
+    std::vector<std::string> lines;
+    if (!read_lines(path, lines)) return kErrFile;
+    if (lines.size() < 2) return kErrOptics;
+    {
+        const auto d = ddyn_impl::tokenize(lines[1].c_str(), 12);
+        double v[5];
+        if (d.size() < 6) return kErrOptics;
+        for (int k = 0; k < 5; ++k)
+            if (!to_double(d[k + 1], v[k])) return kErrOptics;
+        o.n_wav = (int)v[0]; o.n_theta = (int)v[1]; o.n_sizes = (int)v[2];
+        o.dmin = v[3] * 1e-6; o.dmax = v[4] * 1e-6;
+    }
+    if (o.n_wav < 2 || o.n_theta < 2 || o.n_sizes < 2 || o.n_wav > (1 << 20) || o.n_theta > (1 << 16) ||
+        o.n_sizes > (1 << 20) || (size_t)o.n_sizes * o.n_wav * o.n_theta > ((size_t)1 << 31))
+        return kErrOptics;
+    const size_t nw = (size_t)o.n_wav, nt = (size_t)o.n_theta, ns = (size_t)o.n_sizes;
+    o.w.assign(nw, 0.0); o.s.assign(ns, 0.0);
+    o.Qabs.assign(ns * nw, 0.0); o.Qsca.assign(ns * nw, 0.0); o.Qpf.assign(ns * nt * nw, 0.0);
+    size_t j = 0, iw = 0, ip = 0;   // sizes seen, wavelength rows and phase values of the current size
+    std::vector<size_t> rows(ns, 0), pfs(ns, 0);
+    for (size_t i = 2; i < lines.size(); ++i) {
+        const std::string& ln = lines[i];
+        const auto d = ddyn_impl::tokenize(ln.c_str(), 12);
+        if (!ln.empty() && ln[0] == '#') {
+            if (j >= ns || d.size() < 2) return kErrOptics;
+            double v;
+            if (!to_double(d[1], v)) return kErrOptics;
+            o.s[j++] = v * 1e-6;
+            iw = ip = 0;
+            continue;
+        }
+        if (j == 0) continue;
+        if (d.size() == 3) {
+            double v[3];
+            for (int k = 0; k < 3; ++k)
+                if (!to_double(d[k], v[k])) return kErrOptics;
+            if (iw >= nw) return kErrOptics;
+            if (j == 1) o.w[iw] = v[0] * 1e-6;
+            o.Qabs[(j - 1) * nw + iw] = v[1];
+            o.Qsca[(j - 1) * nw + iw] = v[2];
+            rows[j - 1] = ++iw;
+        } else if (d.size() == 10) {
+            if (ip + 10 > nt * nw) return kErrOptics;
+            for (int k = 0; k < 10; ++k) {
+                double v;
+                if (!to_double(d[k], v)) return kErrOptics;
+                o.Qpf[(j - 1) * nt * nw + ip + k] = v;
+            }
+            ip += 10;
+            pfs[j - 1] = ip;
+        }
+    }
+    if (j != ns) return kErrOptics;
+    for (size_t k = 0; k < ns; ++k)
+        if (rows[k] != nw || pfs[k] != nt * nw) return kErrOptics;
+    for (size_t k = 1; k < ns; ++k)
+        if (!(o.s[k] > o.s[k - 1])) return kErrOptics;
+    for (size_t k = 1; k < nw; ++k)
+        if (!(o.w[k] > o.w[k - 1])) return kErrOptics;
+    return 0;
+}
+
+// The stellar spectrum (src/read_in.cu:377-417): lines "<wavelength [m]>
+// <surface flux>" ('#' lines skipped), the flux scaled to the observer at
+// distance cfg->dist by (R_star / dist)^2.
+int read_spectrum(const char* path, const ddyn_config& c, ddyn_optics& o)
+{
+    std::vector<std::string> lines;
+    if (!read_lines(path, lines)) return kErrFile;
+    const double scale = std::pow(c.rstar * kAU, 2) / std::pow(c.dist, 2);
+    for (const auto& ln : lines) {
+        if (!ln.empty() && ln[0] == '#') continue;
+        const auto d = ddyn_impl::tokenize(ln.c_str(), 2);
+        double w, F;
+        if (d.size() < 2 || !to_double(d[0], w) || !to_double(d[1], F)) continue;
+        o.spec_w.push_back(w);
+        o.spec_F.push_back(F * scale);
+    }
+    if (o.spec_w.size() < 2) return kErrOptics;
+    for (size_t k = 1; k < o.spec_w.size(); ++k)
+        if (!(o.spec_w[k] > o.spec_w[k - 1])) return kErrOptics;
+    return 0;
+}
+
+// Index ip of the first table size >= a, clamped so that [ip-1, ip] is an
+// interval of the table (the reference indexes below the table when a equals
+// its smallest size, src/functions.cu:285-286; here the first interval is
+// used, which gives the first row exactly there).
+size_t upper_index(const std::vector<double>& s, double a)
+{
+    size_t ip = 0;
+    while (ip < s.size() && s[ip] < a) ++ip;
+    if (ip == 0) ip = 1;
+    if (ip >= s.size()) ip = s.size() - 1;
+    return ip;
+}
+
+// log-log interpolation between (x0, y0) and (x1, y1) at x, as the reference
+// writes it: exp(log y0 + log(x/x0) log(y1/y0) / log(x1/x0)).  That form is
+// undefined where an efficiency is zero (a grain that does not scatter): equal
+// values are returned as they are, and a zero at one end is interpolated
+// linearly in log x.
+double loglog(double x, double x0, double x1, double y0, double y1)
+{
+    if (y0 == y1) return y0;
+    const double t = std::log(x / x0) / std::log(x1 / x0);
+    if (!(y0 > 0.0) || !(y1 > 0.0)) return y0 + t * (y1 - y0);
+    return std::exp(std::log(y0) + t * std::log(y1 / y0));
+}
+
+// Planck B_lambda (src/functions.cu:390-397).
+double planck(double T, double wvl)
+{
+    return (2 * kPlanck * kC * kC) / (std::pow(wvl, 5) * (std::exp(kPlanck * kC / (wvl * kBoltz * T)) - 1));
+}
+
+// The seeded random stream of particle generation: Mitchell and Moore's
+// additive lagged-Fibonacci generator X[n] = X[n-24] + X[n-55] mod 2^64
+// (Knuth, The Art of Computer Programming vol. 2, 3.2.2, Algorithm A), its
+// 55 words filled from the seed by SplitMix64 (Steele, Lea and Flood, OOPSLA
+// 2014) with one word forced odd, so the period is at least 2^55 - 1.
+// Uniform deviates are the top 53 bits, offset by half a unit: in (0, 1),
+// never 0 or 1.  The reference draws from Numerical Recipes' ran3 instead, so
+// the particles are the same distributions, not the same numbers.
+class Rng {
+public:
+    explicit Rng(int64_t seed)
+    {
+        uint64_t z = (uint64_t)seed;
+        for (int i = 0; i < 55; ++i) {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t x = z;
+            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+            y_[i] = x ^ (x >> 31);
+        }
+        y_[0] |= 1ull;
+        j_ = 23; k_ = 54;
+        for (int i = 0; i < 55 * 8; ++i) next();
+    }
+    uint64_t next()
+    {
+        const uint64_t v = y_[k_] + y_[j_];
+        y_[k_] = v;
+        j_ = j_ ? j_ - 1 : 54;
+        k_ = k_ ? k_ - 1 : 54;
+        return v;
+    }
+    double uniform() { return ((double)(next() >> 11) + 0.5) * 0x1.0p-53; }
+    // Box-Muller: sqrt(-2 ln u1) cos(2 pi u2), the normal deviate the reference forms
+    double normal()
+    {
+        const double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(kPI2 * u2);
+    }
+
+private:
+    uint64_t y_[55];
+    int j_, k_;
+};
+
+struct Sink {
+    int64_t n = 0, cap;
+    double *x, *y, *z, *vx, *vy, *vz, *N, *mu;
+    int32_t* id;
+    bool add(const double* r, double Nv, double m, int32_t idv)
+    {
+        if (n >= cap) return false;
+        x[n] = r[0]; y[n] = r[1]; z[n] = r[2]; vx[n] = r[3]; vy[n] = r[4]; vz[n] = r[5];
+        N[n] = Nv; mu[n] = m; id[n] = idv;
+        ++n;
+        return true;
+    }
+};
+
+double bin_beta(const float* beta, int32_t n_bins, int32_t id)
+{
+    double b = (id >= 0 && id < n_bins && beta) ? (double)beta[id] : 0.0;
+    return b >= 0.5 ? 0.0 : b;   // such grains leave at once: launched on a Kepler orbit (functions.cu:185)
+}
+
+// genorb_disk (src/functions.cu:157-191): e = |normal| sigma_e (< 1), a with
+// a^(2+gamma) uniform between Rin/(1-e) and Rout/(1+e) (surface density ~
+// r^gamma), i = |normal| atan(dh/dr), node, periapsis and true anomaly uniform.
+void orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta, Rng& g, double* r)
+{
+    double e;
+    do { e = std::fabs(g.normal()) * D.sigmae; } while (e >= 1.0);
+    const double amin = D.Rin / (1.0 - e), amax = D.Rout / (1.0 + e);
+    const double u = g.uniform();
+    double a;
+    if (D.gammas == -2.0) {   // the power law's limit: a log-uniform (the reference divides by zero)
+        a = amin * std::exp(u * std::log(amax / amin));
+    } else {
+        const double p = 2.0 + D.gammas;
+        a = std::pow(u * (std::pow(amax, p) - std::pow(amin, p)) + std::pow(amin, p), 1.0 / p);
+    }
+    const double i = std::fabs(g.normal()) * D.dhdr;
+    const double nu = kPI2 * g.uniform(), w = kPI2 * g.uniform(), o = kPI2 * g.uniform();
+    ddyn_impl::coe2rv(c.pm[0], m, a, e, i, w, o, nu, beta, r);
+}
+
+// genorb_blob (src/functions.cu:194-232): a = a_blob + normal sigma (> 0), the
+// blob's e, i, w, Omega, true anomaly nu_blob + normal sigma / a; for a
+// collision cloud a velocity kick of vkepp % of the orbital speed in an
+// isotropic direction.
+void orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta, Rng& g, double* r)
+{
+    double a;
+    do { a = B.a + g.normal() * B.sigma; } while (a <= 0.0);
+    const double nu = B.nu + g.normal() * B.sigma / a;
+    ddyn_impl::coe2rv(c.pm[0], m, a, B.ecc, B.inc, B.w, B.o, nu, beta, r);
+    if (B.collcloud == 0) {
+        const double voff = B.vkepp * 0.01 * std::sqrt(r[3] * r[3] + r[4] * r[4] + r[5] * r[5]);
+        const double ct = -1.0 + 2.0 * g.uniform();
+        const double st = std::sqrt(1.0 - ct * ct);
+        const double gam = kPI2 * g.uniform();
+        r[3] += voff * st * std::cos(gam);
+        r[4] += voff * st * std::sin(gam);
+        r[5] += voff * ct;
+    }
+}
+
+// getbod (src/functions.cu:235-240): the next body of the mass ladder.  The
+// interval [mmin, mmax] below the current mmax holds one body of
+// dN/dm = C m^-eta (ome = 1 - eta); its mass is the interval's mass.  eta = 1
+// and eta = 2 take the logarithmic forms of the integrals (the reference
+// divides by zero there).  Returns mu = G M of the body.
+double next_body(long double& mmax, double C, double eta)
+{
+    const long double ome = 1.0L - (long double)eta;
+    long double mmin, mass;
+    if (eta == 1.0) mmin = mmax * std::exp(-1.0L / (long double)C);
+    else mmin = std::pow(std::pow(mmax, ome) - ome / (long double)C, 1.0L / ome);
+    if (eta == 2.0) mass = (long double)C * std::log(mmax / mmin);
+    else mass = (long double)C * (std::pow(mmax, 1.0L + ome) - std::pow(mmin, 1.0L + ome)) / (1.0L + ome);
+    mmax = mmin;
+    return (double)(mass / kMSun * kGrav);
+}
+
+// The dust mass bins of one disk or blob (src/grid.cu:136-189): its mass
+// range snapped to the global grid, Ngrid bins of ratio exp(f), Npg tracers
+// per bin.
+struct DustBins {
+    double mmin = 0.0;
+    int ngrid = 0, npg = 0;
+};
+
+DustBins dust_bins(const ddyn_config& c, double dmmax, double mmin_body, int64_t nsmall)
+{
+    DustBins b;
+    if (nsmall <= 0 || !(c.mmin > 0.0) || !(c.f > 0.0)) return b;
+    const double mx = c.mmin * std::exp(std::round(std::log(dmmax / c.mmin) / c.f) * c.f);
+    b.mmin = c.mmin * std::exp(std::round(std::log(mmin_body / c.mmin) / c.f) * c.f);
+    b.ngrid = (int)std::round(std::log(mx / b.mmin) / std::log(kMgrid) + 1);
+    if (b.ngrid < 1) { b.ngrid = 0; return b; }
+    b.npg = (int)std::round((double)nsmall / (double)b.ngrid);
+    return b;
+}
+
+// Tracer weight of mass bin [in, ax] of dN/dm = C m^-eta (src/grid.cu:156-158).
+double bin_number(double C, double eta, double in, double ax)
+{
+    if (eta == 0) return C * (ax - in);
+    if (eta == 1) return C * std::log(ax / in);
+    return C * (std::pow(ax, 1.0 - eta) - std::pow(in, 1.0 - eta)) / (1.0 - eta);
+}
+
+template <class F>
+int dust_of(const ddyn_config& c, const ddyn_optics* o, const DustBins& b, double C, double eta, const float* beta,
+            int32_t n_bins, Rng& g, Sink& out, F orbit)
+{
+    const double dmin = o ? o->dmin : 0.0, dmax = o ? o->dmax : std::numeric_limits<double>::infinity();
+    for (int l = 0; l < b.ngrid; ++l) {
+        double m = b.mmin * std::exp((double)l * c.f);
+        const double size = std::pow(m * 3.0 / 4.0 / kPI / c.rho, 1.0 / 3.0);
+        int32_t id = (int32_t)std::round(std::log(m / c.mmin) / c.f);
+        if (size < dmin) id = -1;
+        if (size - dmax > 1e-9) id = -1;
+        const double in = b.mmin * std::exp(((double)l - 0.5) * c.f);
+        const double ax = b.mmin * std::exp(((double)l + 0.5) * c.f);
+        m *= kGrav / kMSun;
+        const double N = bin_number(C, eta, in, ax);
+        const double bt = bin_beta(beta, n_bins, id);
+        for (int k = 0; k < b.npg; ++k) {
+            double r[6];
+            orbit(m, bt, g, r);
+            if (!out.add(r, N / b.npg, m, id)) return kErrInterp;
+        }
+    }
+    return 0;
+}
+
+double norm_c(double Mtot, double eta, double mmax, double mmin)
+{
+    if (eta == 1) return Mtot / (mmax - mmin);
+    if (eta == 2) return Mtot / std::log(mmax / mmin);
+    return Mtot / ((std::pow(mmax, 2.0 - eta) - std::pow(mmin, 2.0 - eta)) / (2.0 - eta));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ddyn_optics_read(const char* comp_path, const char* spec_path, const ddyn_config* cfg, ddyn_optics** out)
+{
+    if (!out) return kErrOptics;
+    *out = nullptr;
+    if (!comp_path || !spec_path || !cfg) return kErrFile;
+    ddyn_optics* o = new ddyn_optics();
+    int rc = read_composition(comp_path, *o);
+    if (!rc) rc = read_spectrum(spec_path, *cfg, *o);
+    if (rc) { delete o; return rc; }
+    *out = o;
+    return 0;
+}
+
+void ddyn_optics_free(ddyn_optics* o) { delete o; }
+
+int ddyn_optics_info(const ddyn_optics* o, int32_t* n_wav, int32_t* n_theta, int32_t* n_sizes, double* dmin,
+                     double* dmax)
+{
+    if (!o) return kErrOptics;
+    if (n_wav) *n_wav = o->n_wav;
+    if (n_theta) *n_theta = o->n_theta;
+    if (n_sizes) *n_sizes = o->n_sizes;
+    if (dmin) *dmin = o->dmin;
+    if (dmax) *dmax = o->dmax;
+    return 0;
+}
+
+int ddyn_set_dust_grid(ddyn_config* c, const ddyn_optics* o)
+{
+    if (!c || !o) return kErrOptics;
+    const double kappa = 4.0 * kPI * c->rho / 3.0;
+    for (int i = 0; i < c->n_disks; ++i) {
+        ddyn_disk& D = c->disks[i];
+        if (D.amin < o->dmin) { D.amin = o->dmin; D.mmin = kappa * std::pow(D.amin, 3.0); }
+        D.dmax = D.amax > o->dmax ? o->dmax : D.amax;
+        D.dmmax = kappa * std::pow(D.dmax, 3.0);
+    }
+    for (int i = 0; i < c->n_blobs; ++i) {
+        ddyn_blob& B = c->blobs[i];
+        if (B.amin < o->dmin) { B.amin = o->dmin; B.mmin = kappa * std::pow(B.amin, 3.0); }
+        B.dmax = B.amax > o->dmax ? o->dmax : B.amax;
+        B.dmmax = kappa * std::pow(B.dmax, 3.0);
+    }
+    double amin, amax;
+    if (c->n_disks > 0) { amin = c->disks[0].amin; amax = c->disks[0].dmax; }
+    else if (c->n_blobs > 0) { amin = c->blobs[0].amin; amax = c->blobs[0].dmax; }
+    else { amin = o->dmin; amax = o->dmax; }
+    for (int i = 1; i < c->n_disks; ++i) {
+        amin = std::fmin(amin, c->disks[i].amin);
+        amax = std::fmax(amax, c->disks[i].dmax);
+    }
+    for (int i = 0; i < c->n_blobs; ++i) {
+        amin = std::fmin(amin, c->blobs[i].amin);
+        amax = std::fmax(amax, c->blobs[i].dmax);
+    }
+    c->amin = amin;
+    c->amax = amax;
+    c->mmin = kappa * std::pow(amin, 3.0);
+    c->mmax = kappa * std::pow(amax, 3.0);
+    c->n_grid = (int)(std::log(c->mmax / c->mmin) / std::log(kMgrid) + 1.0);
+    if (c->n_grid < 2 || c->n_grid > kMaxBins) return kErrInterp;
+    c->f = std::log(c->mmax / c->mmin) / ((double)c->n_grid - 1.0);
+    // normalisations after the size ranges are final (confCdisks/confCblobs)
+    for (int i = 0; i < c->n_disks; ++i)
+        c->disks[i].C = norm_c(c->disks[i].Mtot, c->disks[i].eta, c->disks[i].mmax, c->disks[i].mmin);
+    for (int i = 0; i < c->n_blobs; ++i)
+        c->blobs[i].C = norm_c(c->blobs[i].Mtot, c->blobs[i].eta, c->blobs[i].mmax, c->blobs[i].mmin);
+    c->n_wav = o->n_wav;
+    c->n_theta = o->n_theta;
+    return 0;
+}
+
+double ddyn_outer_radius(const ddyn_config* c)
+{
+    double R = 0.0;
+    for (int i = 0; i < c->n_disks; ++i) R = std::fmax(R, c->disks[i].Rout);
+    for (int i = 0; i < c->n_blobs; ++i)
+        R = std::fmax(R, c->blobs[i].a * (1.0 + c->blobs[i].ecc) + c->blobs[i].sigma);
+    for (int i = 0; i < c->n_planets; ++i)
+        R = std::fmax(R, std::sqrt(c->px[i] * c->px[i] + c->py[i] * c->py[i] + c->pz[i] * c->pz[i]));
+    return 2.0 * R;
+}
+
+int ddyn_grain_tables(const ddyn_config* c, const ddyn_optics* o, int32_t n_max, float* beta, float* betasw,
+                      float* q, double* size, double* Rmin_au, double* lum)
+{
+    if (!c || !o) return kErrOptics;
+    const int n = c->n_grid;
+    if (n < 2 || n > kMaxBins || n > n_max || !(c->amax > c->amin) || !(c->amin > 0.0)) return kErrInterp;
+    const size_t nw = (size_t)o->n_wav, nt = (size_t)o->n_theta;
+    // interpolate_dust_to_grid: sizes of the grid, optical data log-log in size
+    const double fa = std::log(c->amax / c->amin) / ((double)n - 1.0);
+    std::vector<double> a(n), Qa((size_t)n * nw), Qs((size_t)n * nw), Qp((size_t)n * nt * nw);
+    for (int i = 0; i < n; ++i) {
+        a[i] = c->amin * std::exp((double)i * fa);
+        if (i == n - 1 && (a[i] - o->dmax) > 1e-12) return kErrInterp;
+        if (i == n - 1 && (a[i] - o->dmax) < 1e-12 && (a[i] - o->dmax) > 0) a[i] = o->dmax;
+        const size_t ip = upper_index(o->s, a[i]), im = ip - 1;
+        const double s0 = o->s[im], s1 = o->s[ip];
+        for (size_t j = 0; j < nw; ++j) {
+            Qa[i * nw + j] = loglog(a[i], s0, s1, o->Qabs[im * nw + j], o->Qabs[ip * nw + j]);
+            Qs[i * nw + j] = loglog(a[i], s0, s1, o->Qsca[im * nw + j], o->Qsca[ip * nw + j]);
+            for (size_t k = 0; k < nt; ++k)
+                Qp[nw * (nt * i + k) + j] =
+                    loglog(a[i], s0, s1, o->Qpf[nw * (nt * im + k) + j], o->Qpf[nw * (nt * ip + k) + j]);
+        }
+    }
+    // interpolate_stellar_to_grid: the stellar flux at the optical wavelengths
+    std::vector<double> in(nw);
+    const size_t nsp = o->spec_w.size();
+    for (size_t j = 0; j < nw; ++j) {
+        const double wj = o->w[j];
+        if (wj < o->spec_w[0]) {
+            in[j] = 0.0;
+        } else if (wj > o->spec_w[nsp - 1]) {
+            in[j] = o->spec_F[nsp - 1] * std::pow(o->spec_w[nsp - 1] / wj, 4.0);   // Rayleigh-Jeans tail
+        } else {
+            size_t ip = 0;
+            while (ip < nsp - 1 && o->spec_w[ip] <= wj) ++ip;
+            if (ip == 0) ip = 1;
+            const size_t im = ip - 1;
+            const double F0 = o->spec_F[im], F1 = o->spec_F[ip];
+            in[j] = (F0 != 0 && F1 != 0) ? loglog(wj, o->spec_w[im], o->spec_w[ip], F0, F1) : 0.0;
+        }
+    }
+    // calculate_beta: radiation pressure efficiency averaged over the spectrum
+    double star = 0.0;
+    for (size_t j = 0; j + 1 < nw; ++j) star += 0.5 * (o->w[j + 1] - o->w[j]) * (in[j + 1] + in[j]);
+    const double L = star * 4.0 * kPI * std::pow(c->dist, 2.0) / kLsunW;
+    const double m0 = c->pm[0];
+    const double swconst = 3.0 * c->mloss * kCd * kMSun * kYear * c->vsw / 32.0 / kPI / m0 / c->rho / kAU / kAU / kAU;
+    std::vector<double> Qpr(nw);
+    for (int i = 0; i < n; ++i) {
+        double b = 0.0;
+        if (c->radQ == 1 && star > 0.0) {
+            for (size_t j = 0; j < nw; ++j) {
+                double g = 0.0, S = 0.0, cm = -1.0;
+                for (size_t k = 1; k < nt; ++k) {
+                    const double cp = (double)k * 2.0 / ((double)nt - 1.0) - 1.0;
+                    const double p0 = Qp[nw * (nt * i + k - 1) + j], p1 = Qp[nw * (nt * i + k) + j];
+                    S += 0.5 * (p0 + p1) * (cp - cm);
+                    g += 0.5 * (cm * p0 + cp * p1) * (cp - cm);
+                    cm = cp;
+                }
+                g = S != 0.0 ? g / S : 0.0;
+                Qpr[j] = (Qa[i * nw + j] + Qs[i * nw + j] * (1.0 - g)) * in[j];
+            }
+            double Qsum = 0.0;
+            for (size_t j = 0; j + 1 < nw; ++j) Qsum += 0.5 * (o->w[j + 1] - o->w[j]) * (Qpr[j + 1] + Qpr[j]);
+            b = 0.57 * Qsum / star * (L / m0 * kGrav) / a[i] / 1e6 / c->rho / 1e-3;
+        }
+        if (beta) beta[i] = (float)b;
+        if (betasw) betasw[i] = (float)(c->SWQ == 1 ? swconst / a[i] : 0.0);
+        if (q) q[i] = (float)(c->kqe * a[i] * 1e9);
+        if (size) size[i] = a[i];
+    }
+    // calculate_dust_temp: the sublimation radius of the most easily heated bin
+    const double Rmax = ddyn_outer_radius(c) * kAU;
+    double Rmin = Rmax;
+    for (int k = 0; k < n; ++k) {
+        double sk = 0.0, gf = 0.0;
+        for (size_t i = 1; i < nw; ++i) {
+            sk += 0.5 * (o->w[i] - o->w[i - 1]) * (Qa[k * nw + i - 1] * in[i - 1] + Qa[k * nw + i] * in[i]) / kPI;
+            gf += 0.5 * (o->w[i] - o->w[i - 1]) *
+                  (Qa[k * nw + i - 1] * planck(kTsub, o->w[i - 1]) + Qa[k * nw + i] * planck(kTsub, o->w[i]));
+        }
+        sk *= std::pow(c->dist, 2.0);
+        const double Rt = 0.5 * std::sqrt(sk / gf);
+        if (Rt < Rmin) Rmin = Rt;
+    }
+    if (Rmin_au) *Rmin_au = Rmin / kAU;
+    if (lum) *lum = L;
+    return 0;
+}
+
+int ddyn_generate_counts(const ddyn_config* c, int64_t* n_mass, int64_t* n_dust)
+{
+    if (!c) return kErrInterp;
+    int64_t nm = 0, nd = 0;
+    for (int j = 0; j < c->n_disks; ++j) {
+        nm += c->disks[j].Ngrav;
+        const DustBins b = dust_bins(*c, c->disks[j].dmmax, c->disks[j].mmin, c->disks[j].Nsmall);
+        nd += (int64_t)b.ngrid * b.npg;
+    }
+    for (int j = 0; j < c->n_blobs; ++j) {
+        nm += c->blobs[j].Ngrav;
+        const DustBins b = dust_bins(*c, c->blobs[j].dmmax, c->blobs[j].mmin, c->blobs[j].Nsmall);
+        nd += (int64_t)b.ngrid * b.npg;
+    }
+    if (n_mass) *n_mass = nm;
+    if (n_dust) *n_dust = nd;
+    return 0;
+}
+
+int ddyn_generate(const ddyn_config* c, const ddyn_optics* o, const float* beta, int32_t n_bins, int64_t n_all,
+                  double* x, double* y, double* z, double* vx, double* vy, double* vz, double* N, double* mu,
+                  int32_t* id)
+{
+    if (!c || !x || !y || !z || !vx || !vy || !vz || !N || !mu || !id) return kErrInterp;
+    Sink out{0, n_all, x, y, z, vx, vy, vz, N, mu, id};
+    // star and planets first (src/grid.cu:102-106)
+    for (int j = 0; j < c->n_planets; ++j) {
+        const double r[6] = {c->px[j], c->py[j], c->pz[j], c->pvx[j], c->pvy[j], c->pvz[j]};
+        if (!out.add(r, 1.0, c->pm[j], -1)) return kErrInterp;
+    }
+    Rng g(c->seed);
+    // gravitating bodies of the disks, then of the blobs: the mass ladder from each one's mmax down
+    for (int j = 0; j < c->n_disks; ++j) {
+        const ddyn_disk& D = c->disks[j];
+        long double mmax = D.mmax;
+        for (int64_t l = 0; l < D.Ngrav; ++l) {
+            const double m = next_body(mmax, D.C, D.eta);
+            double r[6];
+            orbit_disk(*c, D, m, 0.0, g, r);
+            if (!out.add(r, 1.0, m, -1)) return kErrInterp;
+        }
+    }
+    for (int j = 0; j < c->n_blobs; ++j) {
+        const ddyn_blob& B = c->blobs[j];
+        long double mmax = B.mmax;
+        for (int64_t l = 0; l < B.Ngrav; ++l) {
+            const double m = next_body(mmax, B.C, B.eta);
+            double r[6];
+            orbit_blob(*c, B, m, 0.0, g, r);
+            if (!out.add(r, 1.0, m, -1)) return kErrInterp;
+        }
+    }
+    // massless grains: every disk's mass bins, then every blob's
+    int rc;
+    for (int j = 0; j < c->n_disks; ++j) {
+        const ddyn_disk& D = c->disks[j];
+        const DustBins b = dust_bins(*c, D.dmmax, D.mmin, D.Nsmall);
+        auto orb = [&](double m, double bt, Rng& gg, double* r) { orbit_disk(*c, D, m, bt, gg, r); };
+        if ((rc = dust_of(*c, o, b, D.C, D.eta, beta, n_bins, g, out, orb))) return rc;
+    }
+    for (int j = 0; j < c->n_blobs; ++j) {
+        const ddyn_blob& B = c->blobs[j];
+        const DustBins b = dust_bins(*c, B.dmmax, B.mmin, B.Nsmall);
+        auto orb = [&](double m, double bt, Rng& gg, double* r) { orbit_blob(*c, B, m, bt, gg, r); };
+        if ((rc = dust_of(*c, o, b, B.C, B.eta, beta, n_bins, g, out, orb))) return rc;
+    }
+    return out.n == n_all ? 0 : kErrInterp;
+}
+
+int ddyn_restart_ids(const ddyn_config* c, int64_t n_all, const double* mu, int32_t* id)
+{
+    if (!c || !mu || !id) return kErrInterp;
+    for (int64_t i = 0; i < n_all; ++i) {
+        if (i < c->n_planets) { id[i] = -1; continue; }
+        const double m = mu[i] * kMSun / kGrav;
+        if (!(m > 0.0) || !(c->mmin > 0.0) || !(c->f > 0.0)) { id[i] = -1; continue; }
+        const double r = std::round(std::log(m / c->mmin) / c->f);
+        id[i] = (r < 0.0 || r > (double)(c->n_grid - 1)) ? -1 : (int32_t)r;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -80,6 +80,27 @@ def lib():
         L.ddyn_read_datafile.restype = ctypes.c_int
         L.ddyn_read_datafile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Config),
                                          ctypes.POINTER(RunInfo)] + [_PD] * 8 + [_PI, _PD]
+        L.ddyn_optics_read.restype = ctypes.c_int
+        L.ddyn_optics_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Config),
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.ddyn_optics_free.restype = None
+        L.ddyn_optics_free.argtypes = [ctypes.c_void_p]
+        L.ddyn_optics_info.restype = ctypes.c_int
+        L.ddyn_optics_info.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(_I32)] * 3 + [_PD] * 2
+        L.ddyn_set_dust_grid.restype = ctypes.c_int
+        L.ddyn_set_dust_grid.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p]
+        L.ddyn_outer_radius.restype = ctypes.c_double
+        L.ddyn_outer_radius.argtypes = [ctypes.POINTER(Config)]
+        L.ddyn_grain_tables.restype = ctypes.c_int
+        L.ddyn_grain_tables.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, ctypes.c_int32] + [_PF] * 3 + \
+            [_PD] * 3
+        L.ddyn_generate_counts.restype = ctypes.c_int
+        L.ddyn_generate_counts.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]
+        L.ddyn_generate.restype = ctypes.c_int
+        L.ddyn_generate.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, _PF, ctypes.c_int32, ctypes.c_int64] + \
+            [_PD] * 8 + [_PI]
+        L.ddyn_restart_ids.restype = ctypes.c_int
+        L.ddyn_restart_ids.argtypes = [ctypes.POINTER(Config), ctypes.c_int64, _PD, _PI]
         _lib = L
     return _lib
 
@@ -147,3 +168,83 @@ def read_datafile(path):
     st = DiskState(n - nd, nd, arrs["x"], arrs["y"], arrs["z"], arrs["vx"], arrs["vy"], arrs["vz"], arrs["mu"],
                    arrs["N"], ids, beta_p=np.ascontiguousarray(arrs["beta"][n - nd:]) if nd else None)
     return cfg, run, st
+
+
+class Optics:
+    """The grain composition table and stellar spectrum of a run (ddyn_optics_read)."""
+
+    def __init__(self, comp_path, spec_path, cfg):
+        self.ptr = ctypes.c_void_p()
+        rc = lib().ddyn_optics_read(str(comp_path).encode(), str(spec_path).encode(), ctypes.byref(cfg),
+                                    ctypes.byref(self.ptr))
+        if rc:
+            raise HostIOError(rc, f"optics_read({comp_path}, {spec_path})")
+
+    def info(self):
+        nw, nt, ns = _I32(), _I32(), _I32()
+        dmin, dmax = ctypes.c_double(), ctypes.c_double()
+        lib().ddyn_optics_info(self.ptr, ctypes.byref(nw), ctypes.byref(nt), ctypes.byref(ns), ctypes.byref(dmin),
+                               ctypes.byref(dmax))
+        return {"n_wav": nw.value, "n_theta": nt.value, "n_sizes": ns.value, "dmin": dmin.value, "dmax": dmax.value}
+
+    def close(self):
+        if self.ptr:
+            lib().ddyn_optics_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_dust_grid(cfg, optics):
+    rc = lib().ddyn_set_dust_grid(ctypes.byref(cfg), optics.ptr)
+    if rc:
+        raise HostIOError(rc, "set_dust_grid")
+
+
+def outer_radius(cfg):
+    return lib().ddyn_outer_radius(ctypes.byref(cfg))
+
+
+def grain_tables(cfg, optics):
+    """Per-bin float beta, beta_sw, q, sizes [m], Rmin [AU] and L [Lsun] of the dust grid."""
+    n = cfg.n_grid
+    beta, betasw, q = (np.zeros(n, dtype=np.float32) for _ in range(3))
+    size = np.zeros(n)
+    rmin, lum = ctypes.c_double(), ctypes.c_double()
+    rc = lib().ddyn_grain_tables(ctypes.byref(cfg), optics.ptr, n, _p(beta, _PF), _p(betasw, _PF), _p(q, _PF),
+                                 _p(size), ctypes.byref(rmin), ctypes.byref(lum))
+    if rc:
+        raise HostIOError(rc, "grain_tables")
+    return {"beta": beta, "betasw": betasw, "q": q, "size": size, "Rmin": rmin.value, "lum": lum.value}
+
+
+def generate(cfg, optics=None, beta=None):
+    """Star, planets and the particles of the disks and blobs as a DiskState."""
+    nm, nd = _I64(), _I64()
+    lib().ddyn_generate_counts(ctypes.byref(cfg), ctypes.byref(nm), ctypes.byref(nd))
+    ng = cfg.n_planets + nm.value
+    st = make_state(ng, nd.value)
+    b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float32)
+    rc = lib().ddyn_generate(ctypes.byref(cfg), None if optics is None else optics.ptr, _p(b, _PF),
+                             0 if b is None else len(b), st.n_all,
+                             *[_p(getattr(st, k)) for k in ("x", "y", "z", "vx", "vy", "vz", "N", "mu")],
+                             _p(st.id, _PI))
+    if rc:
+        raise HostIOError(rc, "generate")
+    return st
+
+
+def restart_ids(cfg, st):
+    ids = np.zeros(st.n_all, dtype=np.int32)
+    lib().ddyn_restart_ids(ctypes.byref(cfg), st.n_all, _p(np.ascontiguousarray(st.mu)), _p(ids, _PI))
+    return ids

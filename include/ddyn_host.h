@@ -1,9 +1,10 @@
 /*
  * ddyn_host.h -- host-side envelope of the dynamical hot path (libddyn_host.so).
  *
- * Pure C++ (no GPU): the .param reader, the time-step rule, and the data-file
+ * Pure C++ (no GPU): the .param reader, the time-step rule, the data-file
  * writer/reader, with the record layout of the reference so files move both
- * ways between the two codes.
+ * ways between the two codes, and the set-up of a .param run (optical data,
+ * dust grid, grain tables, particle generation).
  *   ddyn_read_param      replaces read_in_param      (src/read_in.cu:508-733)
  *                        with add_star, add_planets, add_disks, add_blobs and
  *                        recalc_COM (src/read_in.cu:63-234, 344-375)
@@ -98,6 +99,74 @@ int ddyn_write_datafile(const char *path, const char *version, const ddyn_config
 int ddyn_read_datafile(const char *path, const char *version, ddyn_config *cfg, ddyn_run_info *run,
                        double *x, double *y, double *z, double *vx, double *vy, double *vz,
                        double *N, double *mu, int32_t *id, double *beta);
+
+/* ---------------------------------------------------------------------------
+ * Set-up of a .param run (ddyn_setup.cpp): optical data, the dust size grid,
+ * the grain coefficient tables and the particles of the disks and blobs.
+ * ------------------------------------------------------------------------- */
+
+/* The grain composition table and the stellar spectrum (replaces
+ * add_optical_data and add_specfile, src/read_in.cu:377-506).  comp_path: the
+ * .param record 12 file (first line skipped; then n_wav, n_theta, the number of
+ * sizes and the size range in um as tokens 1-5 of the second line; per size a
+ * "# <size um>" line, n_wav lines "<wavelength um> <Qabs> <Qsca>" and the phase
+ * function, theta-major, ten values per line).  spec_path: lines "<wavelength m>
+ * <surface flux>", scaled to the observer by (rstar / dist)^2 (cfg's rstar and
+ * dist).  Returns 39 if a file cannot be opened, 42 if it is not well formed. */
+typedef struct ddyn_optics ddyn_optics;
+int  ddyn_optics_read(const char *comp_path, const char *spec_path, const ddyn_config *cfg, ddyn_optics **out);
+void ddyn_optics_free(ddyn_optics *o);
+int  ddyn_optics_info(const ddyn_optics *o, int32_t *n_wav, int32_t *n_theta, int32_t *n_sizes, double *dmin,
+                      double *dmax);
+
+/* The dust size grid of a .param run (replaces detminmax, confCdisks and
+ * confCblobs, src/read_in.cu:236-342): each disk's and blob's grain sizes
+ * clipped to the optical range (amin, mmin, dmax, dmmax), the global grid amin,
+ * amax, mmin, mmax, n_grid (mass ratio 1.1 between bins) and f, the
+ * normalisations C, n_wav and n_theta.  Returns 54 if n_grid is outside
+ * [2, 2700] (the device tables hold 2,700 bins). */
+int ddyn_set_dust_grid(ddyn_config *cfg, const ddyn_optics *o);
+
+/* Outer radius [AU] of the system, twice the largest of the disks' Rout, the
+ * blobs' a (1 + e) + sigma and the planets' distances (Grid::Rmax,
+ * src/grid.cu:757-764): the data file marks grains beyond it. */
+double ddyn_outer_radius(const ddyn_config *cfg);
+
+/* The grain coefficient tables of the grid's n_grid size bins (replaces
+ * interpolate_dust_to_grid, interpolate_stellar_to_grid and calculate_beta,
+ * src/functions.cu:243-388) and the sublimation radius (calculate_dust_temp,
+ * src/grid.cu:754-792): beta (radiation pressure over gravity), beta_sw
+ * (stellar-wind drag), q (charge factor of the Lorentz force) as the float
+ * tables ddyn_upload_bin_tables takes, size [m] (each may be NULL; n_max >=
+ * n_grid entries), Rmin [AU] and the luminosity [Lsun] (may be NULL). */
+int ddyn_grain_tables(const ddyn_config *cfg, const ddyn_optics *o, int32_t n_max, float *beta, float *betasw,
+                      float *q, double *size, double *Rmin_au, double *lum);
+
+/* Particle counts of the disks and blobs: gravitating bodies (sum of Ngrav) and
+ * grain tracers (per disk or blob, bins x round(Nsmall / bins)); needs the dust
+ * grid (ddyn_set_dust_grid) when grains are asked for. */
+int ddyn_generate_counts(const ddyn_config *cfg, int64_t *n_mass, int64_t *n_dust);
+
+/* The particles of a .param run (replaces Grid::setup's loops with getbod,
+ * genorb_disk and genorb_blob, src/grid.cu:99-189, src/functions.cu:157-240),
+ * n_all = n_planets + n_mass + n_dust entries in reference order and units:
+ * star and planets; each disk's, then each blob's gravitating bodies down the
+ * mass ladder of dN/dm = C m^-eta; then each disk's and blob's grains, Npg
+ * tracers per mass bin with N the bin's number per tracer and id the bin.
+ * beta (n_bins floats, from ddyn_grain_tables) reduces the grains' orbital
+ * speeds as coe2rv does; o gives the optical size range (grains outside it get
+ * id -1; may be NULL).  Orbits follow the reference's distributions (surface
+ * density ~ r^gamma, |normal| eccentricities and inclinations, uniform angles;
+ * Gaussian blobs with an optional collisional velocity kick) from this
+ * library's own seeded generator: the same laws, not the reference's numbers. */
+int ddyn_generate(const ddyn_config *cfg, const ddyn_optics *o, const float *beta, int32_t n_bins, int64_t n_all,
+                  double *x, double *y, double *z, double *vx, double *vy, double *vz, double *N, double *mu,
+                  int32_t *id);
+
+/* Size-bin ids of a restart from the masses (Grid::read_in, src/grid.cu:285-287):
+ * round(log(m / mmin) / f), -1 beyond the grid, for massless bodies and for the
+ * star and planets. */
+int ddyn_restart_ids(const ddyn_config *cfg, int64_t n_all, const double *mu, int32_t *id);
 
 #ifdef __cplusplus
 }
