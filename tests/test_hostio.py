@@ -198,3 +198,48 @@ def test_restart_with_wind_drag_and_grains_is_refused(tmp_path):
                            timeout=60)
         print(r.returncode, r.stderr)
         assert r.returncode == 2 and "beta_sw and charge tables" in r.stderr
+
+
+def test_datafile_writer_matches_reference_layout_byte_for_byte(tmp_path):
+    """The library's writer against tests/golden/datafile_tiny.dat, the text of
+    the reference's fprintf sequence (src/grid.cu:441-673) for a tiny system
+    with a disk, a blob, a massive body and grains beyond the outer radius,
+    inside the star and without a size bin (restated in
+    tests/golden/make_datafile_golden.py)."""
+    from diskdyn_amd.system import DiskState
+    from tests.golden import make_datafile_golden as G
+    c, run_d, s, beta_bins = G.tiny_system()
+    cfg = hostio.Config()
+    for k in ("verb", "gpuid", "gl", "t_gl", "t_write", "t_sed", "t_fits", "t_end", "n_step", "teff", "rstar",
+              "Bstar", "Wz", "mloss", "vsw", "radQ", "SWQ", "FLQ", "rho", "kqe", "dist", "fov", "PA", "inc",
+              "longitude", "xsize", "ysize", "amin", "amax", "mmin", "mmax", "n_grid", "f", "n_wav", "n_theta",
+              "n_planets", "n_disks", "n_blobs", "seed"):
+        setattr(cfg, k, c[k])
+    for k in ("sptype", "name_stub", "specfile", "comp"):
+        setattr(cfg, k, c[k].encode())
+    cfg.nwavs = len(c["wavs"])
+    for i, v in enumerate(c["wavs"]):
+        cfg.wavs[i] = v
+    for k, v in c["disk"].items():
+        setattr(cfg.disks[0], k, v)
+    for k, v in c["blob"].items():
+        setattr(cfg.blobs[0], k, v)
+    for k, arr in (("x", "px"), ("y", "py"), ("z", "pz"), ("vx", "pvx"), ("vy", "pvy"), ("vz", "pvz"), ("m", "pm")):
+        for i, v in enumerate(c["planets"][k]):
+            getattr(cfg, arr)[i] = v
+    run = hostio.RunInfo()
+    for k, v in run_d.items():
+        setattr(run, k, v)
+    n = len(s["x"])
+    run.n_all = n
+    st = DiskState(n - run.n_dust, run.n_dust, *[np.ascontiguousarray(s[k]) for k in ("x", "y", "z", "vx", "vy", "vz",
+                                                                                      "mu", "N")], s["id"])
+    out = tmp_path / "tiny.dat"
+    hostio.write_datafile(out, cfg, st, run, beta_bins=beta_bins)
+    got = out.read_bytes()
+    want = open(G.FIXTURE, "rb").read()
+    assert want == G.expected_text(c, run_d, s, beta_bins).encode()
+    if got != want:
+        for i, (a, b) in enumerate(zip(got.decode().splitlines(), want.decode().splitlines())):
+            assert a == b, f"line {i + 1}: {a!r} != {b!r}"
+    assert got == want
