@@ -44,10 +44,19 @@ namespace {
 
 constexpr int kWG = 256;            // threads per workgroup (4 waves of 64)
 constexpr int kTile = 256;          // sources per LDS tile in the massive pair kernels
-constexpr int kDustSplit = 4;       // grain kernel: waves per workgroup, each over 1/4 of the swarm sources
-constexpr int kDustWG = 64 * kDustSplit;   // 64 grains per workgroup, one per lane
-constexpr int kDustTile = 256;      // grain kernel: sources per FP32 partial and close-pair test
-constexpr int kTargetWGs = 65536;   // pair-kernel workgroups wanted per launch: short workgroups keep
+#ifndef DDYN_DUST_SPLIT
+#define DDYN_DUST_SPLIT 4
+#endif
+#ifndef DDYN_DUST_TILE
+#define DDYN_DUST_TILE 128
+#endif
+constexpr int kDustSplit = DDYN_DUST_SPLIT;   // grain kernel: waves per workgroup, each over a share of the swarm
+constexpr int kDustWG = 64 * kDustSplit;      // 64 grains per workgroup, one per lane
+constexpr int kDustTile = DDYN_DUST_TILE;     // grain kernel: sources per FP32 partial and close-pair test
+#ifndef DDYN_PAIR_TARGET_WGS
+#define DDYN_PAIR_TARGET_WGS 65536
+#endif
+constexpr int kTargetWGs = DDYN_PAIR_TARGET_WGS;   // pair-kernel workgroups wanted per launch: short workgroups keep
                                     // the launch's tail short (measured on the 65,537- and
                                     // 1,048,577-body swarms, 16,384 to 65,536)
 constexpr int kMinSlice = 1024;     // fewest sources per slice (bounds the partials' traffic)
@@ -438,16 +447,19 @@ __device__ inline GrainCoef grain_coef(const DustArgs& A, double bet, double bet
 
 // Star term with radiation pressure, PR and stellar-wind drag and the Lorentz
 // force (src/gpu_functions.cu:679-718).  Returns true if the grain sublimates.
-template <bool RSQ>
-__device__ inline bool star_term(const DustArgs& A, const double4 s0, double px, double py,
-                                 double pz, double ux, double uy, double uz, const GrainCoef& k,
-                                 double& ax, double& ay, double& az)
+// P(c) and U(c) give coordinate c of the grain's stage position and velocity.
+// LEAN (the grain kernel, whose VGPR peak this is) keeps the scheduler from
+// overlapping the three divisions, which holds fewer registers live; the
+// operations and results are the same.
+template <bool RSQ, bool LEAN, class Pos, class Vel>
+__device__ inline bool star_term_at(const DustArgs& A, const double4 s0, Pos P, Vel U, const GrainCoef& k,
+                                    double& ax, double& ay, double& az)
 {
-    const double dx = px - s0.x, dy = py - s0.y, dz = pz - s0.z;
+    const double dx = P(0) - s0.x, dy = P(1) - s0.y, dz = P(2) - s0.z;
     const double RR = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
     if (RR <= A.Rmin2) return true;
     const double m0 = s0.w;
-    const double du = __builtin_fma(dz, uz, __builtin_fma(dy, uy, dx * ux));
+    const double du = __builtin_fma(dz, U(2), __builtin_fma(dy, U(1), dx * U(0)));
     double RR3 = 0.0, iR2 = 0.0, iR3 = 0.0, Fg, Fb1, Fb2;
     if (RSQ) {   // DDYN_FP64_RSQ: powers of one refined 1/R
         const double i1 = rsqrt_fp64(RR);
@@ -458,24 +470,35 @@ __device__ inline bool star_term(const DustArgs& A, const double4 s0, double px,
         Fb2 = du * iR2;
     } else {
         RR3 = RR * __builtin_sqrt(RR);
+        if (LEAN) __builtin_amdgcn_sched_barrier(0);
         Fg = k.gfac * m0 / RR3;
+        if (LEAN) __builtin_amdgcn_sched_barrier(0);
         Fb1 = (A.SWQ == 1) ? m0 / RR * k.drag : k.drag / RR;
+        if (LEAN) __builtin_amdgcn_sched_barrier(0);
         Fb2 = du / RR;
+        if (LEAN) __builtin_amdgcn_sched_barrier(0);
     }
-    const double Fdx = Fb1 * __builtin_fma(Fb2, dx, ux);
-    const double Fdy = Fb1 * __builtin_fma(Fb2, dy, uy);
-    const double Fdz = Fb1 * __builtin_fma(Fb2, dz, uz);
-    double Flx = 0.0, Fly = 0.0;
+    ax = __builtin_fma(-Fg, dx, -(Fb1 * __builtin_fma(Fb2, dx, U(0))));
+    ay = __builtin_fma(-Fg, dy, -(Fb1 * __builtin_fma(Fb2, dy, U(1))));
+    az = __builtin_fma(-Fg, dz, -(Fb1 * __builtin_fma(Fb2, dz, U(2))));
     if (A.FLQ == 1) {
+        // the Lorentz force is added last, so the gravity and drag terms are no
+        // longer live while it is formed
         const double B3 = A.Bstar * A.Rstar * A.Rstar * A.Rstar;
         const double Bz = RSQ ? B3 * iR3 : B3 / RR3;
-        Flx = k.qe * __builtin_fma(-A.Wz, dx, uy) * Bz / k.m;
-        Fly = -k.qe * __builtin_fma(A.Wz, dy, ux) * Bz / k.m;
+        ax = ax + k.qe * __builtin_fma(-A.Wz, dx, U(1)) * Bz / k.m;
+        ay = ay + -k.qe * __builtin_fma(A.Wz, dy, U(0)) * Bz / k.m;
     }
-    ax = __builtin_fma(-Fg, dx, -Fdx) + Flx;
-    ay = __builtin_fma(-Fg, dy, -Fdy) + Fly;
-    az = __builtin_fma(-Fg, dz, -Fdz);
     return false;
+}
+
+template <bool RSQ>
+__device__ inline bool star_term(const DustArgs& A, const double4 s0, double px, double py,
+                                 double pz, double ux, double uy, double uz, const GrainCoef& k,
+                                 double& ax, double& ay, double& az)
+{
+    const double p[3] = {px, py, pz}, u[3] = {ux, uy, uz};
+    return star_term_at<RSQ, false>(A, s0, [&](int c) { return p[c]; }, [&](int c) { return u[c]; }, k, ax, ay, az);
 }
 
 // FP64 pull of sources [j0, j1) on the grain; `soft` adds eps^2.  Sets `hit`
@@ -513,38 +536,105 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(4))) const f4v cf4v;
 typedef __attribute__((address_space(4))) const d4v cd4v;
 
-// Per-workgroup grain records.  Wave 0 owns the RK4 bookkeeping; the other
-// waves read the current stage position from here and post their partial
-// accelerations.  Keeping this state in LDS (19.5 KiB) instead of registers
-// lets the kernel run at 7 waves per SIMD (72 VGPRs) without scratch.
-constexpr int kDustWavesPerEU = 7;
+// Per-workgroup grain records in LDS: the RK4 bookkeeping of the 64 grains,
+// the direct terms (star and planets) of the current stage and every wave's
+// partial swarm acceleration.  Keeping this state in LDS instead of registers
+// keeps the source loop lean (VGPRs for eight waves per SIMD).
+#ifndef DDYN_DUST_WAVES_PER_EU
+#define DDYN_DUST_WAVES_PER_EU 7
+#endif
+#ifndef DDYN_DUST_UNROLL
+#define DDYN_DUST_UNROLL 8
+#endif
+#ifndef DDYN_DUST_F32_RSQ_DIRECT
+#define DDYN_DUST_F32_RSQ_DIRECT 0
+#endif
+#ifndef DDYN_DUST_PREFETCH
+#define DDYN_DUST_PREFETCH 1
+#endif
+#ifndef DDYN_DUST_GROUP
+#define DDYN_DUST_GROUP 4
+#endif
+#ifndef DDYN_DUST_LEAD_CUT_F32
+#define DDYN_DUST_LEAD_CUT_F32 3
+#endif
+#ifndef DDYN_DUST_LEAD_CUT_RSQ
+#define DDYN_DUST_LEAD_CUT_RSQ 1
+#endif
+#ifndef DDYN_DUST_LEAD_CUT_F64
+#define DDYN_DUST_LEAD_CUT_F64 0
+#endif
+constexpr int kDustGroup = DDYN_DUST_GROUP;   // grain kernel: sources per scalar-load group
+static_assert(DDYN_DUST_TILE % (2 * DDYN_DUST_GROUP) == 0, "tile must hold whole pairs of groups");
+static_assert(kDustSplit >= 2, "the grain kernel needs a lead wave and at least one other");
+constexpr int kDustUnit = 2 * kDustGroup;    // sources per share unit: two scalar-load groups
+
+// Sources [b, e) of wave w among ns swarm sources, in units of kDustUnit: the
+// lead (last) wave takes ceil(units / waves) - cut units, the other waves
+// split the rest as evenly as units allow, the larger shares first.  The split
+// depends on the sizes only, so a grain's sums do not depend on its workgroup.
+__device__ inline void dust_share(int ns, int cut, int w, int& b, int& e)
+{
+    const int nu = (ns + kDustUnit - 1) / kDustUnit;
+    const int ul = max(0, min(nu, (nu + kDustSplit - 1) / kDustSplit - cut));
+    const int nr = nu - ul;
+    const int q = nr / (kDustSplit - 1), r = nr % (kDustSplit - 1);
+    int ub = nr, ue = nu;
+    if (w < kDustSplit - 1) {
+        ub = w * q + min(w, r);
+        ue = ub + q + (w < r ? 1 : 0);
+    }
+    b = min(ns, kDustUnit * ub);
+    e = min(ns, kDustUnit * ue);
+}
+constexpr int kDustWavesPerEU = DDYN_DUST_WAVES_PER_EU;
 struct DustShared {
     double x0[3][64], v0[3][64];     // state at the start of the step
     double p[3][64], u[3][64];       // current stage position / velocity
     double k0[3][64], k12[3][64];    // k1 and k2 + k3 (accelerations)
     double r12[3][64];               // r2 + r3 (stage velocities)
-    double coef[4][64];              // GrainCoef: mass, 1 - beta - beta_sw, drag factor, charge factor
-    double a[kDustSplit][3][64];     // per-wave partial accelerations
-    int32_t flag[kDustSplit][64];    // per-wave "grain removed" flags
+    double coef[3][64];              // GrainCoef: 1 - beta - beta_sw, drag factor, charge factor
+    double d[3][64];                 // direct terms of the current stage (star and planets)
+    double a[kDustSplit][3][64];     // per-wave partial swarm accelerations
+    int32_t hit[64];                 // set (never cleared) when a term removes the grain
 };
 
+// A workgroup owns 64 grains, one per lane, in kDustSplit waves; each wave
+// sweeps its share of the swarm sources for all 64 grains (so a launch has
+// many short waves and its drain at the end is short), in FP32 partials over
+// tiles of up to kDustTile sources.  The lead (last) wave also evaluates the
+// FP64 star and planet terms, with a smaller share of the swarm.  After each
+// stage waves 0-2 update one coordinate each: the total acceleration is the
+// direct terms plus the wave partials in wave order.
 template <bool F32, bool ACCEL, bool RSQ = false>
 __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWavesPerEU, 8))) void k_dust(
     DustArgs A)
 {
     __shared__ DustShared S;
+    // the star and planet terms' inverse powers of R: refined hardware rsqrt in FP64_RSQ
+    constexpr bool kDirectRsq = RSQ || (F32 && DDYN_DUST_F32_RSQ_DIRECT);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = blockIdx.x * 64 + lane;
     const int64_t t = (int64_t)A.n_grav + g;
     const bool valid = g < A.n_dust;
     const bool live = valid && (ACCEL || A.N[t] != 0.0);
-    // this wave's share of the swarm sources, in whole tiles
+    // This wave's share of the swarm sources (dust_share).  The last wave leads:
+    // it also evaluates the star and planet terms, and gives up as many
+    // sources as they cost, so the waves meet at the barriers together.
+    constexpr int kLead = kDustSplit - 1;
+    constexpr int kCut = F32 ? DDYN_DUST_LEAD_CUT_F32 : (RSQ ? DDYN_DUST_LEAD_CUT_RSQ : DDYN_DUST_LEAD_CUT_F64);
     const int ns = A.n_grav - A.n_planets;
-    const int per = ((ns + kDustSplit * kDustTile - 1) / (kDustSplit * kDustTile)) * kDustTile;
-    const int jb = A.n_planets + min(ns, wave * per);
-    const int je = A.n_planets + min(ns, (wave + 1) * per);
-    const int nparts = per > 0 ? (ns + per - 1) / per : 1;   // waves holding sources (wave 0 always)
+    int jb, je;
+    dust_share(ns, kCut, wave, jb, je);
+    jb += A.n_planets;
+    je += A.n_planets;
+    unsigned parts = 0;   // waves holding sources
+    for (int w = 0; w < kDustSplit; ++w) {
+        int b, e;
+        dust_share(ns, kCut, w, b, e);
+        parts |= (b < e ? 1u : 0u) << w;
+    }
     const double dt = A.dt, dt2 = dt / 2.0;
     if (wave == 0) {
         const double xs[3] = {valid ? A.x[t] : kFar, valid ? A.y[t] : kFar, valid ? A.z[t] : kFar};
@@ -568,28 +658,42 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
             qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
         }
         const GrainCoef k = grain_coef(A, bet, betsw, qe, m, A.p64[0][0].w);
-        S.coef[0][lane] = k.m; S.coef[1][lane] = k.gfac; S.coef[2][lane] = k.drag; S.coef[3][lane] = k.qe;
+        S.coef[0][lane] = k.gfac; S.coef[1][lane] = k.drag; S.coef[2][lane] = k.qe;
+        S.hit[lane] = 0;
     }
     __syncthreads();
     bool gone = false;
     constexpr int kStages = ACCEL ? 1 : 4;
 #pragma unroll 1
     for (int s = 0; s < kStages; ++s) {
-        const double px = S.p[0][lane], py = S.p[1][lane], pz = S.p[2][lane];
         const bool act = live && !gone;
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        bool hit = false;
-        if (wave == 0 && act) {
+        if (wave == kLead) {
             // star and planets (src/gpu_functions.cu:679-730), FP64
-            const GrainCoef k = {S.coef[1][lane], S.coef[2][lane], S.coef[3][lane], S.coef[0][lane]};
-            hit = star_term<RSQ>(A, A.p64[s][0], px, py, pz, S.u[0][lane], S.u[1][lane], S.u[2][lane], k, ax, ay,
-                            az);
-            grain_fp64<RSQ>(A.p64[s], 1, A.n_planets, px, py, pz, 0.0, ax, ay, az, hit);
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            bool hit = false;
+            if (act) {
+                // the grain's mass enters only the Lorentz force; it is read from
+                // the state rather than kept in the LDS record
+                const GrainCoef k = {S.coef[0][lane], S.coef[1][lane], S.coef[2][lane],
+                                     A.FLQ == 1 ? A.mu[t] : 1.0};
+                hit = star_term_at<kDirectRsq, true>(
+                    A, A.p64[s][0], [&](int c) { return S.p[c][lane]; }, [&](int c) { return S.u[c][lane]; }, k,
+                    ax, ay, az);
+                // the stage position is read again for the planets, so it is not
+                // held in registers through the star term (the kernel's VGPR peak)
+                __asm__ volatile("" ::: "memory");
+                grain_fp64<kDirectRsq>(A.p64[s], 1, A.n_planets, S.p[0][lane], S.p[1][lane], S.p[2][lane], 0.0,
+                                       ax, ay, az, hit);
+            }
+            S.d[0][lane] = ax; S.d[1][lane] = ay; S.d[2][lane] = az;
+            if (hit) S.hit[lane] = 1;
         }
+        bool hit = false;
+        S.a[wave][0][lane] = 0.0; S.a[wave][1][lane] = 0.0; S.a[wave][2][lane] = 0.0;
         if (__builtin_amdgcn_readfirstlane(__any(act)) && jb < je) {
             if (F32) {
                 const cf4v* src = (const cf4v*)A.p32[s];
-                const float xf = (float)px, yf = (float)py, zf = (float)pz;
+                const float xf = (float)S.p[0][lane], yf = (float)S.p[1][lane], zf = (float)S.p[2][lane];
                 const float e2f = fmaxf(A.eps2f, kTiny2);
                 for (int j0 = jb; j0 < je; j0 += kDustTile) {
                     const int tl = min(kDustTile, je - j0);
@@ -607,24 +711,65 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
                         fy = __builtin_fmaf(dy, q, fy);
                         fz = __builtin_fmaf(dz, q, fz);
                     };
-                    if (tl == kDustTile) {
-#pragma unroll 8
-                        for (int jj = 0; jj < kDustTile; ++jj) pull(src[j0 + jj]);
+                    if (tl % kDustUnit == 0) {
+#if DDYN_DUST_PREFETCH
+                        // kDustGroup sources per group in scalar registers, two groups
+                        // in turn.  Scalar loads return out of order, so a wait covers
+                        // every load in flight: the first pull of a group comes
+                        // before the next group's loads are issued (its wait then
+                        // covers only its own group), and the group's other pulls
+                        // run while those loads are in flight.
+                        constexpr int G = kDustGroup;
+                        const cf4v* sp = src + j0;
+                        f4v ga[G], gb[G];
+#pragma unroll
+                        for (int u = 0; u < G; ++u) ga[u] = sp[u];
+#pragma unroll 1
+                        for (int jj = 0; jj < tl; jj += 2 * G) {
+                            pull(ga[0]);
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int u = 0; u < G; ++u) gb[u] = sp[jj + G + u];
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int u = 1; u < G; ++u) pull(ga[u]);
+                            pull(gb[0]);
+                            __builtin_amdgcn_sched_barrier(0);
+                            // past the tile's end the last group is loaded again and not used
+                            const int jn = min(jj + 2 * G, tl - G);
+#pragma unroll
+                            for (int u = 0; u < G; ++u) ga[u] = sp[jn + u];
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int u = 1; u < G; ++u) pull(gb[u]);
+                        }
+#else
+#pragma unroll DDYN_DUST_UNROLL
+                        for (int jj = 0; jj < tl; ++jj) pull(src[j0 + jj]);
+#endif
                     } else {
                         for (int jj = 0; jj < tl; ++jj) pull(src[j0 + jj]);
                     }
                     const bool redo = __any(act && rmin < A.thr2f);
                     if (act) {
+                        // the FP64 accumulation of the wave's tiles lives in LDS
                         if (!redo) {
-                            ax += (double)fx; ay += (double)fy; az += (double)fz;
+                            S.a[wave][0][lane] += (double)fx;
+                            S.a[wave][1][lane] += (double)fy;
+                            S.a[wave][2][lane] += (double)fz;
                         } else {
-                            grain_fp64<false>(A.p64[s], j0, j0 + tl, px, py, pz, A.eps2, ax, ay, az, hit);
+                            double ax = S.a[wave][0][lane], ay = S.a[wave][1][lane], az = S.a[wave][2][lane];
+                            grain_fp64<false>(A.p64[s], j0, j0 + tl, S.p[0][lane], S.p[1][lane], S.p[2][lane],
+                                              A.eps2, ax, ay, az, hit);
+                            S.a[wave][0][lane] = ax; S.a[wave][1][lane] = ay; S.a[wave][2][lane] = az;
                         }
                     }
                 }
             } else {
                 const cd4v* src = (const cd4v*)A.p64[s];
                 const double e2 = A.eps2;
+                const double px = S.p[0][lane], py = S.p[1][lane], pz = S.p[2][lane];
+                double ax = 0.0, ay = 0.0, az = 0.0;
 #pragma unroll 4
                 for (int j = jb; j < je; ++j) {
                     const d4v sj = src[j];
@@ -643,68 +788,54 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
                     ay = __builtin_fma(-f, ey, ay);
                     az = __builtin_fma(-f, ez, az);
                 }
+                if (act) { S.a[wave][0][lane] = ax; S.a[wave][1][lane] = ay; S.a[wave][2][lane] = az; }
             }
         }
-        S.a[wave][0][lane] = ax; S.a[wave][1][lane] = ay; S.a[wave][2][lane] = az;
-        S.flag[wave][lane] = hit ? 1 : 0;
+        // Removal is permanent, so one flag per grain serves every stage: a lane
+        // that is still active at this stage found no hit at an earlier one.
+        // Waves only ever store 1 into it.
+        if (hit) S.hit[lane] = 1;
         __syncthreads();
-        bool any_hit = false;
-#pragma unroll
-        for (int w = 0; w < kDustSplit; ++w) any_hit = any_hit || (S.flag[w][lane] != 0);
-        gone = gone || (act && any_hit);
-        if (wave == 0 && valid) {
-            // total: wave partials in wave order (wave 0 holds the star and planets)
-            double tot[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double v = S.a[0][c][lane];
-                for (int w = 1; w < nparts; ++w) v += S.a[w][c][lane];
-                tot[c] = v;
-            }
+        gone = gone || (act && S.hit[lane] != 0);
+        if (wave < 3 && valid) {
+            // coordinate c = wave: the direct terms, then the wave partials in wave order
+            const int c = wave;
+            double tot = S.d[c][lane];
+            for (int w = 0; w < kDustSplit; ++w)
+                if ((parts >> w) & 1u) tot += S.a[w][c][lane];
             if (ACCEL) {
-                A.ox[g] = gone ? 0.0 : tot[0];
-                A.oy[g] = gone ? 0.0 : tot[1];
-                A.oz[g] = gone ? 0.0 : tot[2];
-                if (A.removed) A.removed[g] = gone ? 1 : 0;
+                double* const O[3] = {A.ox, A.oy, A.oz};
+                O[c][g] = gone ? 0.0 : tot;
+                if (c == 0 && A.removed) A.removed[g] = gone ? 1 : 0;
             } else if (s < 3) {
                 // next stage (src/gpu_functions.cu:737-743, 800-806, 863-869)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double x0 = S.x0[c][lane], v0 = S.v0[c][lane], uo = S.u[c][lane];
-                    if (s == 0) {
-                        S.k0[c][lane] = tot[c];
-                        S.p[c][lane] = __builtin_fma(v0, dt2, x0);
-                        S.u[c][lane] = __builtin_fma(tot[c], dt2, v0);
-                    } else if (s == 1) {
-                        const double r2v = __builtin_fma(tot[c], dt2, v0);
-                        S.k12[c][lane] = tot[c];
-                        S.p[c][lane] = __builtin_fma(uo, dt2, x0);
-                        S.r12[c][lane] = uo + r2v;
-                        S.u[c][lane] = r2v;
-                    } else {
-                        S.k12[c][lane] = S.k12[c][lane] + tot[c];
-                        S.p[c][lane] = __builtin_fma(uo, dt, x0);
-                        S.u[c][lane] = __builtin_fma(tot[c], dt, v0);
-                    }
+                const double x0 = S.x0[c][lane], v0 = S.v0[c][lane], uo = S.u[c][lane];
+                if (s == 0) {
+                    S.k0[c][lane] = tot;
+                    S.p[c][lane] = __builtin_fma(v0, dt2, x0);
+                    S.u[c][lane] = __builtin_fma(tot, dt2, v0);
+                } else if (s == 1) {
+                    const double r2v = __builtin_fma(tot, dt2, v0);
+                    S.k12[c][lane] = tot;
+                    S.p[c][lane] = __builtin_fma(uo, dt2, x0);
+                    S.r12[c][lane] = uo + r2v;
+                    S.u[c][lane] = r2v;
+                } else {
+                    S.k12[c][lane] = S.k12[c][lane] + tot;
+                    S.p[c][lane] = __builtin_fma(uo, dt, x0);
+                    S.u[c][lane] = __builtin_fma(tot, dt, v0);
                 }
             } else if (live) {
-                // evolve (src/gpu_functions.cu:927-933); removal sets N = 0.  The
-                // empty asm makes the index opaque so that the compiler forms the
-                // six addresses here instead of holding them through the loop.
-                int64_t ti = t;
-                asm volatile("" : "+v"(ti));
+                // evolve (src/gpu_functions.cu:927-933); removal sets N = 0
                 if (gone) {
-                    A.N[ti] = 0.0;
+                    if (c == 0) A.N[t] = 0.0;
                 } else {
                     const double dt6 = dt / 6.0;
+                    const double x0 = S.x0[c][lane], v0 = S.v0[c][lane];
                     double* const X[3] = {A.x, A.y, A.z};
                     double* const V[3] = {A.vx, A.vy, A.vz};
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double x0 = S.x0[c][lane], v0 = S.v0[c][lane];
-                        X[c][ti] = __builtin_fma(dt6, __builtin_fma(2.0, S.r12[c][lane], v0) + S.u[c][lane], x0);
-                        V[c][ti] = __builtin_fma(dt6, __builtin_fma(2.0, S.k12[c][lane], S.k0[c][lane]) + tot[c], v0);
-                    }
+                    X[c][t] = __builtin_fma(dt6, __builtin_fma(2.0, S.r12[c][lane], v0) + S.u[c][lane], x0);
+                    V[c][t] = __builtin_fma(dt6, __builtin_fma(2.0, S.k12[c][lane], S.k0[c][lane]) + tot, v0);
                 }
             }
         }
