@@ -1,10 +1,13 @@
 // ddyn_main.cpp -- command-line driver of the dynamical hot path on one GPU.
 //
-//   ddyn -i <file.param> [options]   start from a .param file (star + planets)
+//   ddyn -i <file.param> [options]   start from a .param file: star, planets and the
+//                                    particles its disks and blobs generate
 //   ddyn -m <file.dat>   [options]   continue from a data file (every particle)
 //
-// It replaces the dynamics of the reference's main + Grid::evolve
-// (src/ddyn.cu:37-111, src/grid.cu:344-439): the same time step (det_step),
+// It replaces the dynamics of the reference's main + Grid::setup / read_in +
+// Grid::evolve (src/ddyn.cu:37-111, src/grid.cu:62-310, 344-439): the optical
+// data and the grain tables of the dust grid (with grains), the particles of
+// the disks and blobs, the same time step (det_step),
 // the same clipping of steps to the data/FITS/SED output times, and data files
 // <stub>_<time>.dat written at the data cadence and at the end.  Imaging and
 // SEDs are not produced; their output times still clip the step so the
@@ -15,6 +18,7 @@
 //          swarm pairs), --r64 <AU> (FP32-force form: swarm pairs closer than
 //          this are evaluated in FP64; default 10 eps, or 0.01 AU without
 //          softening), --rmin <AU> (sublimation radius), --device <n>, --quiet.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -83,31 +87,64 @@ int main(int argc, char** argv)
     State st;
     int rc;
     bool param_mode = param != nullptr;
+    // grain coefficient tables of the dust grid (from the optical data) and the
+    // sublimation radius they give
+    std::vector<float> tb, tbsw, tq;
+    double rmin_optics = 0.0;
+    bool have_tables = false;
+    ddyn_optics* optics = nullptr;
+    auto load_tables = [&]() -> int {
+        const size_t n = (size_t)std::max(cfg.n_grid, 0);
+        tb.assign(n, 0.f); tbsw.assign(n, 0.f); tq.assign(n, 0.f);
+        double lum = 0.0;
+        const int r = ddyn_grain_tables(&cfg, optics, cfg.n_grid, tb.data(), tbsw.data(), tq.data(), nullptr,
+                                        &rmin_optics, &lum);
+        if (r) return r;
+        if (!quiet)
+            std::printf("ddyn: %d size bins, L = %.4g Lsun, sublimation radius %.5g AU\n", cfg.n_grid, lum,
+                        rmin_optics);
+        have_tables = true;
+        return 0;
+    };
     if (param_mode) {
         if ((rc = ddyn_read_param(param, &cfg))) {
             std::fprintf(stderr, "ddyn: %s: %s\n", param, ddyn_host_error_text(rc));
             return 1;
         }
-        for (int k = 0; k < cfg.n_disks; ++k)
-            if (cfg.disks[k].Ngrav || cfg.disks[k].Nsmall) {
-                std::fprintf(stderr, "ddyn: disk particle generation is not supported; "
-                                     "start from a data file with -m\n");
-                return 2;
+        bool generate = false;
+        for (int k = 0; k < cfg.n_disks; ++k) generate = generate || cfg.disks[k].Ngrav || cfg.disks[k].Nsmall;
+        for (int k = 0; k < cfg.n_blobs; ++k) generate = generate || cfg.blobs[k].Ngrav || cfg.blobs[k].Nsmall;
+        if (generate) {
+            // optical data, dust grid and grain tables (src/ddyn.cu:78-96, src/grid.cu:66-70)
+            if ((rc = ddyn_optics_read(cfg.comp, cfg.specfile, &cfg, &optics))) {
+                std::fprintf(stderr, "ddyn: optical data %s / spectrum %s: %s\n", cfg.comp, cfg.specfile,
+                             ddyn_host_error_text(rc));
+                return 1;
             }
-        for (int k = 0; k < cfg.n_blobs; ++k)
-            if (cfg.blobs[k].Ngrav || cfg.blobs[k].Nsmall) {
-                std::fprintf(stderr, "ddyn: blob particle generation is not supported; "
-                                     "start from a data file with -m\n");
-                return 2;
+            if ((rc = ddyn_set_dust_grid(&cfg, optics)) || (rc = load_tables())) {
+                std::fprintf(stderr, "ddyn: dust grid: %s\n", ddyn_host_error_text(rc));
+                ddyn_optics_free(optics);
+                return 1;
             }
-        st.n_grav = cfg.n_planets;
-        st.resize(st.n_grav);
-        for (int j = 0; j < cfg.n_planets; ++j) {
-            st.x[j] = cfg.px[j]; st.y[j] = cfg.py[j]; st.z[j] = cfg.pz[j];
-            st.vx[j] = cfg.pvx[j]; st.vy[j] = cfg.pvy[j]; st.vz[j] = cfg.pvz[j];
-            st.mu[j] = cfg.pm[j]; st.N[j] = 1.0;
         }
-        run.n_mass = 0; run.n_dust = 0; run.n_all = st.n_grav;
+        int64_t n_mass = 0, n_dust = 0;
+        ddyn_generate_counts(&cfg, &n_mass, &n_dust);
+        st.n_grav = cfg.n_planets + n_mass;
+        st.n_dust = n_dust;
+        st.resize(st.n_grav + st.n_dust);
+        rc = ddyn_generate(&cfg, optics, have_tables ? tb.data() : nullptr, have_tables ? cfg.n_grid : 0,
+                           st.n_grav + st.n_dust, st.x.data(), st.y.data(), st.z.data(), st.vx.data(), st.vy.data(),
+                           st.vz.data(), st.N.data(), st.mu.data(), st.id.data());
+        ddyn_optics_free(optics);
+        optics = nullptr;
+        if (rc) {
+            std::fprintf(stderr, "ddyn: particle generation: %s\n", ddyn_host_error_text(rc));
+            return 1;
+        }
+        if (!quiet && generate)
+            std::printf("ddyn: generated %lld gravitating bodies and %lld grain tracers\n", (long long)n_mass,
+                        (long long)n_dust);
+        run.n_mass = n_mass; run.n_dust = n_dust; run.n_all = st.n_grav + st.n_dust;
     } else {
         if ((rc = ddyn_read_datafile(model, kVersion, &cfg, &run, nullptr, nullptr, nullptr, nullptr, nullptr,
                                      nullptr, nullptr, nullptr, nullptr, nullptr))) {
@@ -123,23 +160,29 @@ int main(int argc, char** argv)
         }
         st.n_dust = run.n_dust;
         st.n_grav = run.n_all - run.n_dust;
-        // The data file carries beta per grain only; the reference recomputes
-        // beta_sw and the charge from the optical data on restart
-        // (src/grid.cu:296).  Without those tables the wind drag and the
-        // Lorentz force would silently be zero, so such a run is refused.
-        if (st.n_dust > 0 && (cfg.SWQ == 1 || cfg.FLQ == 1)) {
-            std::fprintf(stderr, "ddyn: %s has stellar-wind drag or the Lorentz force on, which need "
-                                 "beta_sw and charge tables that a data file does not carry\n", model);
-            return 2;
+        if (st.n_dust > 0) {
+            // as the reference's restart (src/grid.cu:285-296): the grain tables
+            // from the optical data on the file's dust grid, the bins from the masses
+            if (ddyn_optics_read(cfg.comp, cfg.specfile, &cfg, &optics) == 0 && load_tables() == 0) {
+                ddyn_restart_ids(&cfg, run.n_all, st.mu.data(), st.id.data());
+            } else if (cfg.SWQ == 1 || cfg.FLQ == 1) {
+                // the data file carries beta per grain only: without the optical
+                // data the wind drag and the Lorentz force would silently be zero
+                std::fprintf(stderr, "ddyn: %s has stellar-wind drag or the Lorentz force on, which need "
+                                     "beta_sw and charge tables that a data file does not carry, and the optical "
+                                     "data %s / spectrum %s cannot be read\n", model, cfg.comp, cfg.specfile);
+                ddyn_optics_free(optics);
+                return 2;
+            } else {
+                std::fprintf(stderr, "ddyn: warning: optical data %s / spectrum %s cannot be read: per-grain beta "
+                                     "from the data file%s\n", cfg.comp, cfg.specfile,
+                             rmin > 0.0 ? "" : ", and no sublimation radius (--rmin)");
+            }
+            ddyn_optics_free(optics);
+            optics = nullptr;
         }
-        if (st.n_dust > 0 && rmin <= 0.0)
-            std::fprintf(stderr, "ddyn: warning: no sublimation radius (--rmin): grains are never removed "
-                                 "near the star\n");
     }
-    run.Rmax = 0.0;
-    for (int k = 0; k < cfg.n_disks; ++k) run.Rmax = std::max(run.Rmax, cfg.disks[k].Rout);
-    for (int k = 0; k < cfg.n_blobs; ++k)
-        run.Rmax = std::max(run.Rmax, cfg.blobs[k].a * (1.0 + cfg.blobs[k].ecc) + 3.0 * cfg.blobs[k].sigma);
+    run.Rmax = ddyn_outer_radius(&cfg);
     if (run.Rmax <= 0.0) run.Rmax = 1e300;
 
     ddyn_ctx* ctx = nullptr;
@@ -153,7 +196,7 @@ int main(int argc, char** argv)
     prm.FLQ = cfg.FLQ;
     prm.precision = precision;
     prm.eps = eps;
-    prm.Rmin = rmin;
+    prm.Rmin = rmin > 0.0 ? rmin : (have_tables ? rmin_optics : 0.0);
     prm.vsw = cfg.vsw * kYear / kAU;
     prm.Wz = cfg.Wz;
     prm.Bstar = cfg.Bstar;
@@ -173,7 +216,13 @@ int main(int argc, char** argv)
         ddyn_destroy(ctx);
         return 1;
     }
-    if (st.n_dust) {   // beta per grain from the data file's beta column; no wind or charge tables
+    if (have_tables) {   // the reference's per-bin float tables (src/ddyn.cu:29-32)
+        if (check(ctx, ddyn_upload_bin_tables(ctx, cfg.n_grid, tb.data(), tbsw.data(), tq.data()),
+                  "upload_bin_tables")) {
+            ddyn_destroy(ctx);
+            return 1;
+        }
+    } else if (st.n_dust) {   // beta per grain from the data file's beta column
         std::vector<double> b(st.beta.begin() + st.n_grav, st.beta.end());
         if (check(ctx, ddyn_upload_grain_beta(ctx, b.data(), nullptr), "upload_grain_beta")) {
             ddyn_destroy(ctx);
@@ -186,10 +235,11 @@ int main(int argc, char** argv)
             return 1;
         char name[600];
         std::snprintf(name, sizeof name, "%s_%.2f.dat", cfg.name_stub[0] ? cfg.name_stub : "ddyn", time);
-        const double* bp = st.n_dust ? st.beta.data() + st.n_grav : nullptr;
+        const double* bp = (st.n_dust && !have_tables) ? st.beta.data() + st.n_grav : nullptr;
         int r = ddyn_write_datafile(name, kVersion, &cfg, &run, st.x.data(), st.y.data(), st.z.data(),
                                     st.vx.data(), st.vy.data(), st.vz.data(), st.N.data(), st.mu.data(),
-                                    st.id.data(), nullptr, 0, bp);
+                                    st.id.data(), have_tables ? tb.data() : nullptr, have_tables ? cfg.n_grid : 0,
+                                    bp);
         if (!quiet) std::printf("ddyn: wrote %s\n", name);
         return r;
     };

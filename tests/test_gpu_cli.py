@@ -86,11 +86,13 @@ def test_param_run_and_restart(tmp_path):
     assert np.abs(fin2.pos() - fin.pos()).max() < 1e-6 * np.abs(fin.pos()).max()
 
 
-def test_param_with_generated_particles_is_refused(tmp_path):
+def test_param_with_particles_needs_its_optical_data(tmp_path):
+    """Particles to generate need the optical data the .param names
+    (optical/astrosil.dat here, absent): refused with the reference's code 39."""
     p = tmp_path / "d.param"
     p.write_text(PARAM.replace("-3.5 0.1 -1.0 0 0", "-3.5 0.1 -1.0 0 100"))
     r = subprocess.run([DDYN, "-i", str(p), "--quiet"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "not supported" in r.stderr
+    assert r.returncode == 1 and "optical data" in r.stderr
 
 
 def test_precision_flags_without_swarm(tmp_path):
@@ -112,3 +114,65 @@ def test_precision_flags_without_swarm(tmp_path):
     _, _, a = hostio.read_datafile(texts[0])
     _, _, b = hostio.read_datafile(texts[2])
     assert np.abs(b.pos() - a.pos()).max() <= 1e-8 * np.abs(a.pos()).max()
+
+
+def _write_optics(tmp_path):
+    from tests import opticslib as OL
+    spec = tmp_path / "star.dat"
+    OL.write_spectrum(spec, 5800.0)
+    comp = tmp_path / "gray.dat"
+    OL.gray_table(comp, np.geomspace(0.1, 2000.0, 14), np.geomspace(0.05, 2000.0, 400), qabs=0.7, qsca=0.4,
+                  g_lin=0.6, n_theta=21)
+    return comp, spec
+
+
+def test_param_run_generates_disks_and_blobs(tmp_path):
+    """A .param run whose disk and blob generate particles: 200 gravitating
+    disk bodies (the reference law, no softening), 3,000 disk grains and 1,000
+    blob grains from a collision, with per-bin beta, beta_sw from the optical
+    data, wind drag on.  The first data file holds exactly the generated state
+    (host generator, same seed); the final file matches the oracle stepped from
+    it with the same dt sequence, bin tables and sublimation radius; a -m
+    restart from the middle file, which rebuilds the tables from the optical
+    data and the bins from the masses, reaches the same final state."""
+    from tests import opticslib as OL
+    comp, spec = _write_optics(tmp_path)
+    disks = "30.0 40.0 0.05 0.05 100.0 1.0 3.67 0.5 -1.0 200 3000"
+    blobs = "35.0 0.1 5.0 10.0 20.0 30.0 100.0 1.0 3.67 0.1 0 1000 collision 3.0"
+    p = OL.setup_param(tmp_path / "gen.param", comp, spec, disks, blobs, physics="yes yes no", t_end=1.0)
+    p.write_text(p.read_text().replace("1.0 1.0 1.0\n", "0.5 0.5 0.5\n", 1))
+    r = subprocess.run([DDYN, "-i", str(p)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "generated 200 gravitating bodies" in r.stdout
+    cfg = hostio.read_param(p)
+    opt = hostio.Optics(cfg.comp.decode(), cfg.specfile.decode(), cfg)
+    hostio.set_dust_grid(cfg, opt)
+    tab = hostio.grain_tables(cfg, opt)
+    st0 = hostio.generate(cfg, opt, tab["beta"])
+    assert st0.n_grav == cfg.n_planets + 200 and st0.n_dust > 3500
+    _, run0, f0 = hostio.read_datafile(tmp_path / "setupstub_0.00.dat")
+    assert run0.n_mass == 200 and run0.n_dust == st0.n_dust
+    assert np.abs(f0.pos() - st0.pos()).max() <= 1e-8 * np.abs(st0.pos()).max()
+    assert np.array_equal(f0.id[st0.n_grav:], st0.id[st0.n_grav:])
+    # the oracle with the same tables, dt sequence and sublimation radius
+    st0.beta_t, st0.betasw_t, st0.q_t = tab["beta"], tab["betasw"], tab["q"]
+    prm = hostio.params_from_config(cfg, Rmin=tab["Rmin"])
+    dt = hostio.det_step(cfg, st0)
+    seq = dt_sequence(cfg.t_write, cfg.t_fits, cfg.t_sed, cfg.t_end, dt)
+    ref = st0.copy()
+    for d in seq:
+        O.step(prm, ref, d, 1)
+    _, run, fin = hostio.read_datafile(tmp_path / "setupstub_1.00.dat")
+    assert abs(run.time - 1.0) < 1e-12
+    assert np.array_equal(fin.N == 0, ref.N == 0)
+    err = np.abs(fin.pos() - ref.pos()).max() / np.abs(ref.pos()).max()
+    print(f"[cli gen] {len(seq)} steps, final position error {err:.2e} (file precision %.8e)")
+    assert err < 1e-8
+    (tmp_path / "restart").mkdir()
+    r = subprocess.run([DDYN, "-m", str(tmp_path / "setupstub_0.50.dat"), "--quiet"], cwd=tmp_path / "restart",
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    _, _, fin2 = hostio.read_datafile(tmp_path / "restart" / "setupstub_1.00.dat")
+    assert np.abs(fin2.pos() - fin.pos()).max() < 1e-6 * np.abs(fin.pos()).max()
