@@ -52,7 +52,7 @@ STAGE_BYTES_PER_BODY = 720 + 4 * 16
 GRAIN_BYTES_PER_GRAIN = 120
 
 
-def workload(name, precision):
+def workload(name, precision, n_dust=None):
     if name == "cfg1":
         w = synth.config1()
         desc = ("small planetary system: 1 Msun star, planets at 5.2 and 9.5 AU, "
@@ -65,9 +65,10 @@ def workload(name, precision):
         w = synth.config2(precision=precision)
         desc = "self-gravitating swarm: 1 Msun star + 65,536 bodies, eps=1e-3 AU, dt=0.02 yr"
     elif name == "cfg3":
-        w = synth.config3(precision=precision)
+        nd = 262144 if n_dust is None else n_dust
+        w = synth.config3(n_dust=nd, precision=precision)
         desc = ("dusty debris ring: 0.5 Msun star, 3e-5 Msun planet, 1,024 planetesimals, "
-                "262,144 grains with radiation/PR/wind drag, dt=0.05 yr")
+                f"{nd:,} grains with radiation/PR/wind drag, dt=0.05 yr")
     elif name == "cfg5":
         w = synth.config5(precision=precision)
         desc = "large self-gravitating disk: 1 Msun star + 1,048,576 bodies, eps=5e-4 AU, dt=0.02 yr"
@@ -191,15 +192,17 @@ def force_error(w, a_gpu, nsample=2048, seed=7):
     return float(np.median(err)), float(err.max())
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (each counter scaled by its
-    reading for a copy kernel of known size), or None if not profiled."""
+def measured_traffic(kernel, workload, n_grav, n_dust):
+    """HBM bytes per launch of `kernel` on `workload` at these sizes from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (each counter
+    scaled by its reading for a copy kernel of known size), or None if not
+    profiled."""
     import glob
     for path in glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json")):
         with open(path) as f:
             d = json.load(f)
-        if d.get("kernel") == kernel and d.get("workload") == workload:
+        if (d.get("kernel") == kernel and d.get("workload") == workload and d.get("n_grav") == n_grav
+                and d.get("n_dust") == n_dust):
             scale_f = d["copy_kib_true"] / d["copy_fetch_kib_raw"]
             scale_w = d["copy_kib_true"] / d["copy_write_kib_raw"]
             return int(round((d["fetch_kib_raw"] * scale_f + d["write_kib_raw"] * scale_w) * 1024))
@@ -357,10 +360,10 @@ def adjoint_run(name, precision=FP32_FORCE):
 DEFAULT_STEPS = {"cfg1": 1000, "cfg2": 200, "cfg3": 500, "cfg4": 400, "cfg5": 20}
 
 
-def measure(name, steps, fp64_steps, warmup, in_order=False, cpu=False):
+def measure(name, steps, fp64_steps, warmup, in_order=False, cpu=False, n_dust=None):
     """All three precision forms of one workload; returns the JSON fields of
     its line (value, roofline, errors, speed-ups, state update, ...)."""
-    w32, desc = workload(name, FP32_FORCE)
+    w32, desc = workload(name, FP32_FORCE, n_dust)
     per_step = 4 * w32.interactions_per_eval()
     t32, kt32, a32, drift32, io32 = timed_run(w32, FP32_FORCE, steps, warmup, in_order)
     print(f"[bench] {name} FP32-force: {steps} steps in {t32:.3f} s", file=sys.stderr, flush=True)
@@ -423,7 +426,7 @@ def measure(name, steps, fp64_steps, warmup, in_order=False, cpu=False):
                          "note": "|dE/E0| of the massive set over the timed steps (each form's own step count)"},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": measured_traffic(kname, w32.name), "launches": nl, "avg_launch_ms": avg_ms,
+                     "traffic": measured_traffic(kname, w32.name, w32.state.n_grav, w32.state.n_dust), "launches": nl, "avg_launch_ms": avg_ms,
                      "flop_per_launch": FLOP_PER_INTERACTION * inter_launch,
                      "note": "achieved = 20 FLOP x interactions per launch / mean HIP-event launch time; "
                              "traffic = calibrated FETCH_SIZE + WRITE_SIZE bytes per launch from profiles/"},
@@ -458,7 +461,12 @@ def main():
                     help="with --adjoint cfg5: FP64 with refined hardware rsqrt in every pair law")
     ap.add_argument("--in-order", action="store_true",
                     help="time every pass with one launch per kernel, in order (kernel profiling runs)")
+    ap.add_argument("--n-dust", type=int, default=None,
+                    help="cfg3 only: grain count of the debris ring (default 262,144, the stated size; "
+                         "other counts are for kernel scaling studies)")
     args = ap.parse_args()
+    if args.n_dust is not None and (args.workload != "cfg3" or args.n_dust < 1):
+        raise SystemExit("--n-dust takes a positive grain count with --workload cfg3")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the swarm would shard by target block, the grains by grain range)", file=sys.stderr)
@@ -470,7 +478,7 @@ def main():
 
     steps = args.steps if args.steps is not None else DEFAULT_STEPS[args.workload]
     k64 = args.fp64_steps if args.fp64_steps is not None else steps
-    m = measure(args.workload, steps, k64, args.warmup, args.in_order, cpu=not args.no_cpu)
+    m = measure(args.workload, steps, k64, args.warmup, args.in_order, cpu=not args.no_cpu, n_dust=args.n_dust)
     out = {
         "metric": "pairwise interactions/s & RK4 steps/s, FP32-force vs FP64 at 1e-5 rel err",
         "value": m.pop("value"),
@@ -488,7 +496,8 @@ def main():
     }
     m.pop("steps")
     out.update(m)
-    if args.workload == "cfg3" and not args.no_also and args.steps is None and not args.in_order:
+    if (args.workload == "cfg3" and not args.no_also and args.steps is None and not args.in_order
+            and args.n_dust is None):
         # the swarm and the disk at reduced step counts: their own throughput,
         # roofline and FP32-force speed-up (no CPU baseline)
         also = {}

@@ -60,6 +60,7 @@ constexpr int kTargetWGs = DDYN_PAIR_TARGET_WGS;   // pair-kernel workgroups wan
                                     // the launch's tail short (measured on the 65,537- and
                                     // 1,048,577-body swarms, 16,384 to 65,536)
 constexpr int kMinSlice = 1024;     // fewest sources per slice (bounds the partials' traffic)
+constexpr int kAdjPullMin = 64;     // more massive bodies than this: grain pulls of the adjoint as a pair kernel
 constexpr double kLightAU = 63239.6717367;   // src/ddyn.cuh:22
 constexpr double kFar = 1.0e15;     // position of padding sources/targets [AU]
 constexpr float kTiny2 = 1.0e-20f;  // FP32 r^2 floor for unsoftened pairs (keeps self pairs finite)
@@ -1257,6 +1258,7 @@ __global__ __launch_bounds__(kWG) void k_vjp_pairs32(const double4* __restrict__
 
 struct AdjArgs {
     int n_grav, n_planets, n_dust, nslice, nwaves;
+    int n_wsum;          // massive bodies whose grain pulls k_adj_dust sums per wave: n_grav, or 1 (the star)
     size_t n_all;
     const double4* p64[4];
     const double* x; const double* y; const double* z;
@@ -1274,7 +1276,10 @@ struct AdjArgs {
     double* Abm;         // 4 stages * 3 * n_grav
     double* Rbg;         // 4 stages * 3 * n_grav: grain pulls on massive stage positions
     double* mug;         // n_grav: grain pulls on massive masses (this step)
-    double* wpart;       // per-wave partials: nwaves * 4 stages * n_grav * 4
+    double* wpart;       // per-wave partials: nwaves * 4 stages * n_wsum * 4
+    double* gR;          // k_adj_pull path (else null): grain stage positions, 4 stages * 3 * n_dust (kFar if inactive)
+    double* gA;          // k_adj_pull path: grain acceleration adjoints, 4 stages * 3 * n_dust (0 if inactive)
+    double* ppart;       // k_adj_pull path: grain-pull slice partials, nslice_g * 4 stages * 4 * n_grav
     const double* part;  // VJP pair partials
     double* g_mu;        // n_grav accumulated
     double* g_beta;      // n_dust accumulated
@@ -1303,7 +1308,11 @@ __device__ inline double wave_sum(double v)
 
 // Grain reverse: one grain per lane; recomputes its four FP64 stages (as k_dust
 // does in the FP64 form), then runs the stage reverse in registers.  The pull
-// of the grain on each massive body is summed per wave and written to wpart.
+// of the grain on the first n_wsum massive bodies is summed per wave and
+// written to wpart.  With n_wsum = 1 (many massive bodies) only the star's
+// pull is, since it carries the drag terms; the grain's stage positions and
+// acceleration adjoints go to gR, gA instead, and k_adj_pull sums the pulls
+// on the other massive bodies as a pair kernel over LDS tiles of grains.
 __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
 {
     const int g = blockIdx.x * kWG + threadIdx.x;
@@ -1371,7 +1380,15 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
         const double* Vs = V[st];
         const double* Av = Ab[st];
         double Rb[3] = {0, 0, 0};
-        double* wp = A.wpart + ((size_t)wave * 4 + st) * n * 4;
+        const size_t nw = (size_t)A.n_wsum;
+        double* wp = A.wpart + ((size_t)wave * 4 + st) * nw * 4;
+        if (A.gR && g < A.n_dust) {
+            const size_t nd = (size_t)A.n_dust;
+            for (int c = 0; c < 3; ++c) {
+                A.gR[(st * 3 + c) * nd + g] = act ? Rs[c] : kFar;
+                A.gA[(st * 3 + c) * nd + g] = act ? Av[c] : 0.0;
+            }
+        }
         {   // star term
             const double4 s0 = P[0];
             const double m0 = s0.w;
@@ -1420,10 +1437,11 @@ __global__ __launch_bounds__(kWG) void k_adj_dust(AdjArgs A)
                 if (act) Rb[c] -= pj.w * JA;
             }
             c4[3] = act ? qA * inv3 : 0.0;
-            for (int c = 0; c < 4; ++c) {
-                const double s = wave_sum(c4[c]);
-                if (lane == 0) wp[(size_t)j * 4 + c] = s;
-            }
+            if (nw > 1)
+                for (int c = 0; c < 4; ++c) {
+                    const double s = wave_sum(c4[c]);
+                    if (lane == 0) wp[(size_t)j * 4 + c] = s;
+                }
         }
         for (int c = 0; c < 3; ++c) {
             r_out[c] += Rb[c];
@@ -1450,12 +1468,12 @@ __global__ __launch_bounds__(kWG) void k_adj_reduce(AdjArgs A)
 {
     const int item = blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);   // (st, j)
     const int lane = threadIdx.x & 63;
-    const int n = A.n_grav;
-    if (item >= 4 * n) return;   // whole waves
-    const int st = item / n, j = item % n;
+    const int n = A.n_grav, nw = A.n_wsum;
+    if (item >= 4 * nw) return;   // whole waves
+    const int st = item / nw, j = item % nw;
     double s[4] = {0, 0, 0, 0};
     for (int w = lane; w < A.nwaves; w += 64) {
-        const double* p = A.wpart + (((size_t)w * 4 + st) * n + j) * 4;
+        const double* p = A.wpart + (((size_t)w * 4 + st) * nw + j) * 4;
         for (int c = 0; c < 4; ++c) s[c] += p[c];
     }
     for (int c = 0; c < 4; ++c) s[c] = wave_sum(s[c]);
@@ -1465,11 +1483,90 @@ __global__ __launch_bounds__(kWG) void k_adj_reduce(AdjArgs A)
         double mt = s[3];
         for (int s2 = 1; s2 < 4; ++s2) {
             double acc = 0.0;
-            for (int w = lane; w < A.nwaves; w += 64) acc += A.wpart[(((size_t)w * 4 + s2) * n + j) * 4 + 3];
+            for (int w = lane; w < A.nwaves; w += 64) acc += A.wpart[(((size_t)w * 4 + s2) * nw + j) * 4 + 3];
             mt += wave_sum(acc);
         }
         if (lane == 0) A.mug[j] = mt;
     }
+}
+
+// Grain pulls on massive bodies j >= 1 (reverse sweep, many massive bodies):
+// targets j, one per lane; sources the grains of slice blockIdx.y at stage
+// blockIdx.z, their stage positions and acceleration adjoints staged through
+// LDS in tiles of kTile.  Per pair (q = r_j - r_g, s2 = |q|^2 + eps^2 for
+// swarm bodies, A the grain's acceleration adjoint) it adds
+// J(q) A = A / s^3 - 3 q (q.A) / s^5, times mu_j at the end, and (q.A) / s^3
+// for mu_j, the same terms that k_adj_dust sums per wave.  Writes
+// ppart[((y * 4 + st) * 4 + c) * n_grav + j].  Inactive grains sit at kFar
+// with A = 0 and add exactly zero.
+__global__ __launch_bounds__(kWG) void k_adj_pull(AdjArgs A, int slice_len)
+{
+    __shared__ double tr[3][kTile];
+    __shared__ double ta[3][kTile];
+    const int st = blockIdx.z;
+    const int j = 1 + blockIdx.x * kWG + threadIdx.x;
+    const bool ok = j < A.n_grav;
+    const double4 pj = ok ? A.p64[st][j] : make_double4(-kFar, -kFar, -kFar, 0.0);
+    const double e2 = (j >= A.n_planets) ? A.eps2 : 0.0;
+    const size_t nd = (size_t)A.n_dust;
+    const int gb = blockIdx.y * slice_len;
+    const int ge = min(gb + slice_len, A.n_dust);
+    const double* R = A.gR + (size_t)st * 3 * nd;
+    const double* Av = A.gA + (size_t)st * 3 * nd;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sm = 0.0;
+    for (int g0 = gb; g0 < ge; g0 += kTile) {
+        const int tl = min(kTile, ge - g0);
+        __syncthreads();
+        if ((int)threadIdx.x < tl) {
+            const size_t g = (size_t)g0 + threadIdx.x;
+            for (int c = 0; c < 3; ++c) {
+                tr[c][threadIdx.x] = R[c * nd + g];
+                ta[c][threadIdx.x] = Av[c * nd + g];
+            }
+        }
+        __syncthreads();
+        for (int gg = 0; gg < tl; ++gg) {
+            const double qx = pj.x - tr[0][gg], qy = pj.y - tr[1][gg], qz = pj.z - tr[2][gg];
+            const double ax = ta[0][gg], ay = ta[1][gg], az = ta[2][gg];
+            const double s2 = __builtin_fma(qz, qz, __builtin_fma(qy, qy, qx * qx)) + e2;
+            const double inv = rsqrt_fp64(s2), inv3 = inv * inv * inv, inv5 = inv3 * inv * inv;
+            const double qA = __builtin_fma(qz, az, __builtin_fma(qy, ay, qx * ax));
+            const double t3 = 3.0 * qA * inv5;
+            sx = __builtin_fma(ax, inv3, __builtin_fma(-qx, t3, sx));
+            sy = __builtin_fma(ay, inv3, __builtin_fma(-qy, t3, sy));
+            sz = __builtin_fma(az, inv3, __builtin_fma(-qz, t3, sz));
+            sm = __builtin_fma(qA, inv3, sm);
+        }
+    }
+    if (ok) {
+        const size_t n = (size_t)A.n_grav;
+        double* p = A.ppart + ((size_t)blockIdx.y * 4 + st) * 4 * n;
+        p[j] = pj.w * sx; p[n + j] = pj.w * sy; p[2 * n + j] = pj.w * sz; p[3 * n + j] = sm;
+    }
+}
+
+// Sums of k_adj_pull's slice partials in slice order: Rbg[st][c][j] and
+// mug[j] (stages in order) for j >= 1.  One wave per massive body: lane l
+// sums slices l, l + 64, ..., and a fixed shuffle tree adds the lane sums.
+__global__ __launch_bounds__(kWG) void k_adj_pull_reduce(AdjArgs A, int nslice_g)
+{
+    const int j = 1 + blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= A.n_grav) return;   // whole waves
+    const size_t n = (size_t)A.n_grav;
+    double mt = 0.0;
+    for (int st = 0; st < 4; ++st) {
+        double s[4] = {0, 0, 0, 0};
+        for (int y = lane; y < nslice_g; y += 64) {
+            const double* p = A.ppart + ((size_t)y * 4 + st) * 4 * n;
+            for (int c = 0; c < 4; ++c) s[c] += p[c * n + j];
+        }
+        for (int c = 0; c < 4; ++c) s[c] = wave_sum(s[c]);
+        if (lane == 0)
+            for (int c = 0; c < 3; ++c) A.Rbg[((size_t)st * 3 + c) * n + j] = s[c];
+        mt += s[3];
+    }
+    if (lane == 0) A.mug[j] = mt;
 }
 
 // Massive stage reverse (stage ST): combine VJP pair partials with the grain
@@ -2482,16 +2579,33 @@ int reverse_tape(ddyn_ctx* c, const double* lam_r, const double* lam_v, double* 
     const int64_t nsteps = c->tape_n, ck = c->tape_ck_every, nseg = seg_count(nsteps, ck);
     const size_t na = (size_t)(c->n_grav + c->n_dust), ng = (size_t)c->n_grav, nd = (size_t)c->n_dust;
     const int nwaves = (int)((nd + kWG - 1) / kWG) * (kWG / 64);
+    // Grain pulls on the massive bodies: summed per grain wave when there are
+    // few massive bodies (or in the one-launch-per-kernel schedule), else as a
+    // pair kernel with massive targets and grain sources (k_adj_pull)
+    const bool pull_pairs = nd > 0 && ng > (size_t)kAdjPullMin && c->fuse_small;
+    const int n_wsum = pull_pairs ? 1 : (int)ng;
+    const int pull_tb = pull_pairs ? (int)((ng - 1 + kWG - 1) / kWG) : 0;
+    int pull_len = 0, pull_nsl = 0;
+    if (pull_pairs) {
+        const int want = std::max(1, kTargetWGs / (4 * pull_tb));
+        pull_len = (int)std::max<size_t>(kMinSlice, (nd + want - 1) / want);
+        pull_len = (pull_len + kTile - 1) / kTile * kTile;
+        pull_nsl = (int)((nd + pull_len - 1) / pull_len);
+    }
     // VJP source split over [0, n_grav)
     const int tblocks = (int)((ng + kWG - 1) / kWG);
     int vsl = std::max(1, std::min((kTargetWGs + tblocks - 1) / tblocks, (int)(ng / kMinSlice)));
     int vlen = (int)(((ng + vsl - 1) / vsl + kTile - 1) / kTile) * kTile;
     vsl = (int)((ng + vlen - 1) / vlen);
-    DevBuf rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart;
+    DevBuf rbb, vbb, Vbm, Abm, Rbg, mug, gmu, gbeta, wpart, vpart, gR, gA, ppart;
     if (dalloc(c, rbb.p, 3 * na) || dalloc(c, vbb.p, 3 * na) ||
         dalloc(c, Vbm.p, 12 * ng) || dalloc(c, Abm.p, 12 * ng) || dalloc(c, Rbg.p, 12 * ng) ||
         dalloc(c, mug.p, ng) || dalloc(c, gmu.p, ng) || dalloc(c, gbeta.p, std::max<size_t>(nd, 1)) ||
-        dalloc(c, wpart.p, (size_t)std::max(nwaves, 1) * 16 * ng) || dalloc(c, vpart.p, (size_t)vsl * 4 * ng))
+        dalloc(c, wpart.p, (size_t)std::max(nwaves, 1) * 16 * (size_t)n_wsum) ||
+        dalloc(c, vpart.p, (size_t)vsl * 4 * ng))
+        return DDYN_ERR_NOMEM;
+    if (pull_pairs && (dalloc(c, gR.p, 12 * nd) || dalloc(c, gA.p, 12 * nd) ||
+                       dalloc(c, ppart.p, (size_t)pull_nsl * 16 * ng)))
         return DDYN_ERR_NOMEM;
     const unsigned bg = (unsigned)((ng + kWG - 1) / kWG);
     DDYN_HIP(c, hipMemcpyAsync(rbb.p, lam_r, 3 * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -2501,7 +2615,8 @@ int reverse_tape(ddyn_ctx* c, const double* lam_r, const double* lam_v, double* 
 
     AdjArgs A{};
     A.n_grav = (int)ng; A.n_planets = c->prm.n_planets; A.n_dust = (int)nd; A.nslice = vsl;
-    A.nwaves = nwaves; A.n_all = na;
+    A.nwaves = nwaves; A.n_wsum = n_wsum; A.n_all = na;
+    A.gR = gR.p; A.gA = gA.p; A.ppart = ppart.p;
     for (int s = 0; s < 4; ++s) A.p64[s] = c->p64[s];
     A.x = c->x; A.y = c->y; A.z = c->z; A.vx = c->vx; A.vy = c->vy; A.vz = c->vz; A.mu = c->mu;
     A.N = c->N; A.id = c->id;
@@ -2529,7 +2644,12 @@ int reverse_tape(ddyn_ctx* c, const double* lam_r, const double* lam_v, double* 
             if (!small) k_adj_init<<<bg, kWG, 0, c->stream>>>(A);
             if (nd) {
                 k_adj_dust<<<(unsigned)((nd + kWG - 1) / kWG), kWG, 0, c->stream>>>(A);
-                k_adj_reduce<<<(unsigned)((4 * ng + kWG / 64 - 1) / (kWG / 64)), kWG, 0, c->stream>>>(A);
+                k_adj_reduce<<<(unsigned)((4 * (size_t)n_wsum + kWG / 64 - 1) / (kWG / 64)), kWG, 0, c->stream>>>(A);
+                if (pull_pairs) {
+                    k_adj_pull<<<dim3((unsigned)pull_tb, (unsigned)pull_nsl, 4), kWG, 0, c->stream>>>(A, pull_len);
+                    k_adj_pull_reduce<<<(unsigned)((ng - 1 + kWG / 64 - 1) / (kWG / 64)), kWG, 0, c->stream>>>(
+                        A, pull_nsl);
+                }
             }
             const int wg = nd ? 1 : 0;
             if (small) {

@@ -22,6 +22,8 @@ CASES = {
     "ring": (lambda: synth.config4(n_dust=256), 20, 6),
     "swarm": (lambda: synth.config2(n_swarm=300, precision=FP64, eps=1e-2), 10, 4),
     "debris": (lambda: synth.config3(n_dust=200, n_swarm=24, precision=FP64, eps=2e-2), 8, 3),
+    # more than 64 massive bodies: the grain pulls run as a pair kernel (k_adj_pull)
+    "debris_many": (lambda: synth.config3(n_dust=700, n_swarm=160, precision=FP64, eps=2e-2), 6, 3),
 }
 
 
@@ -52,7 +54,7 @@ def test_gpu_adjoint_matches_cpu_adjoint(engine_cls, name):
     assert np.abs(fin.pos() - ref.pos()).max() <= 1e-10 * np.abs(ref.pos()).max()
 
 
-@pytest.mark.parametrize("name", ["debris", "swarm"])
+@pytest.mark.parametrize("name", ["debris", "debris_many", "swarm"])
 def test_gpu_adjoint_fp64_rsq_context(engine_cls, name):
     """FP64_RSQ context: the forward pass takes the refined-rsqrt pair
     laws; the gradients still meet the FP64 bar against the CPU adjoint."""
@@ -70,6 +72,32 @@ def test_gpu_adjoint_fp64_rsq_context(engine_cls, name):
         errs["beta"] = _rel(g_beta, c_beta)
     print(f"[adjoint fp64_rsq] {name}", {k: f"{v:.2e}" for k, v in errs.items()})
     assert max(errs.values()) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32_FORCE])
+def test_grain_pull_pairs_match_wave_sums(engine_cls, precision):
+    """The debris ring's 1,026 massive bodies with 2,048 grains: the grain
+    pulls on the massive bodies as a pair kernel over LDS tiles of grains
+    (default schedule) against the per-wave sums of the one-launch-per-kernel
+    schedule and against the CPU adjoint."""
+    w = synth.config3(n_dust=2048, precision=precision)
+    st = w.state
+    nsteps, ck = 2, 1
+    rng = np.random.default_rng(17)
+    lam_r, lam_v = rng.normal(size=(st.n_all, 3)), rng.normal(size=(st.n_all, 3))
+    out = {}
+    for sched in (0, 1):
+        with engine_cls() as eng:
+            eng.upload(st.copy(), replace(w.params, schedule=sched))
+            out[sched] = eng.adjoint(w.dt, nsteps, lam_r, lam_v, ckpt_every=ck)
+    d = {k: _rel(out[0][i], out[1][i]) for i, k in enumerate(("r", "v", "mu", "beta"))}
+    print(f"[grain pulls] pairs vs wave sums, precision={precision}", {k: f"{v:.2e}" for k, v in d.items()})
+    assert max(d.values()) <= 1e-12
+    if precision == FP64:
+        c = O.adjoint(w.params, st, w.dt, nsteps, lam_r, lam_v)
+        e = {k: _rel(out[0][i], c[i]) for i, k in enumerate(("r", "v", "mu", "beta"))}
+        print("   against the CPU adjoint", {k: f"{v:.2e}" for k, v in e.items()})
+        assert max(e.values()) <= 1e-9
 
 
 FUSED_REVERSE = {
