@@ -76,7 +76,10 @@ typedef struct {
                             planets only, <= 32 massive bodies), and with grains run the grain
                             kernel of step k beside the massive stages of step k+1 on a second
                             stream; 1: one launch per kernel, in order, on one stream.
-                            Both give identical results. */
+                            Both give identical results.  In the adjoint with grains and
+                            more than 64 massive bodies, 0 also sums the grains' pulls on
+                            the massive bodies as a pair kernel, 1 per grain wave: those
+                            gradients agree to rounding, not bit for bit. */
     int32_t reserved;    /* must be 0 */
 } ddyn_params;
 
