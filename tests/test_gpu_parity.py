@@ -204,16 +204,20 @@ def test_fused_small_steps_more_workgroups_than_fit_at_once(engine_cls):
     assert pos_rel_err(out[0], ref, slice(0, w.state.n_grav)).max() <= 1e-10
 
 
+@pytest.mark.parametrize("n_dust", [3000, 140000])
 @pytest.mark.parametrize("precision", [FP64, FP32_FORCE, FP64_RSQ])
-def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision):
+def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision, n_dust):
     """Swarm + grains: the grain kernel of step k beside the massive stages of
-    step k+1 (double-buffered stage positions) equals one launch per kernel in
+    step k+1 (double-buffered stage positions; with 131,072 grains or more
+    the grain blocks split over two streams) equals one launch per kernel in
     order, bit for bit, over two calls with grain removals at the first step
     and at the fifth, and the forces at the end state agree."""
-    w = synth.config3(n_dust=3000, n_swarm=700, precision=precision)
+    w = synth.config3(n_dust=n_dust, n_swarm=700, precision=precision)
     st = w.state.copy()
     g0 = st.n_grav
     st.x[g0], st.y[g0], st.z[g0] = st.x[0] + 0.05, st.y[0], st.z[0]   # sublimates (Rmin) at once
+    last = st.n_all - 1                                               # the last block (second stream)
+    st.x[last], st.y[last], st.z[last] = st.x[0], st.y[0] - 0.05, st.z[0]
     # at rest 1 AU from the star: falls inside Rmin after about five steps
     st.x[g0 + 1], st.y[g0 + 1], st.z[g0 + 1] = st.x[0] + 1.0, st.y[0], st.z[0]
     st.vx[g0 + 1], st.vy[g0 + 1], st.vz[g0 + 1] = st.vx[0], st.vy[0], st.vz[0]
@@ -229,13 +233,13 @@ def test_overlapped_grain_stream_matches_in_order_schedule(engine_cls, precision
             eng.download(s)
             a, rem = eng.accel()
         out.append((mid, s, a, rem))
-    print(f"[overlap] precision={precision} removed {int((out[0][1].N == 0).sum())}")
+    print(f"[overlap] precision={precision} n_dust={n_dust} removed {int((out[0][1].N == 0).sum())}")
     for k in ("x", "y", "z", "vx", "vy", "vz", "N"):
         assert np.array_equal(getattr(out[0][0], k), getattr(out[1][0], k)), k
         assert np.array_equal(getattr(out[0][1], k), getattr(out[1][1], k)), k
     assert np.array_equal(out[0][2], out[1][2])
     assert np.array_equal(out[0][3], out[1][3])
-    assert out[0][1].N[g0] == 0.0
+    assert out[0][1].N[g0] == 0.0 and out[0][1].N[last] == 0.0
     assert out[0][0].N[g0 + 1] == 0.0 and out[0][0].x[g0 + 1] != st.x[g0 + 1]
 
 
