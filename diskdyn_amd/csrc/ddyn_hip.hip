@@ -2808,6 +2808,10 @@ int ddyn_synchronize(ddyn_ctx* c)
     if (!c) return DDYN_ERR_ARG;
     DDYN_HIP(c, hipSetDevice(c->device));
     DDYN_HIP(c, hipStreamSynchronize(c->stream));
+    // the grain streams are joined by the main stream at the end of ddyn_step;
+    // a step that stopped on an error may have left work on them
+    DDYN_HIP(c, hipStreamSynchronize(c->gstream));
+    DDYN_HIP(c, hipStreamSynchronize(c->gstream2));
     return DDYN_OK;
 }
 
