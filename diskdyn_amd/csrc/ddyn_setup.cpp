@@ -267,13 +267,19 @@ double bin_beta(const float* beta, int32_t n_bins, int32_t id)
     return b >= 0.5 ? 0.0 : b;   // such grains leave at once: launched on a Kepler orbit (functions.cu:185)
 }
 
+// Draws of a rejection loop before generation gives up (a law whose accepted
+// region has a probability below about 1e-5 per draw is a bad input).
+constexpr int kMaxDraws = 1 << 20;
+
 // genorb_disk (src/functions.cu:157-191): e = |normal| sigma_e (< 1), a with
 // a^(2+gamma) uniform between Rin/(1-e) and Rout/(1+e) (surface density ~
 // r^gamma), i = |normal| atan(dh/dr), node, periapsis and true anomaly uniform.
-void orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta, Rng& g, double* r)
+// Returns false if no e below 1 was drawn.
+bool orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta, Rng& g, double* r)
 {
-    double e;
-    do { e = std::fabs(g.normal()) * D.sigmae; } while (e >= 1.0);
+    double e = 1.0;
+    for (int k = 0; k < kMaxDraws && !(e < 1.0); ++k) e = std::fabs(g.normal()) * D.sigmae;
+    if (!(e < 1.0)) return false;
     const double amin = D.Rin / (1.0 - e), amax = D.Rout / (1.0 + e);
     const double u = g.uniform();
     double a;
@@ -286,16 +292,18 @@ void orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta,
     const double i = std::fabs(g.normal()) * D.dhdr;
     const double nu = kPI2 * g.uniform(), w = kPI2 * g.uniform(), o = kPI2 * g.uniform();
     ddyn_impl::coe2rv(c.pm[0], m, a, e, i, w, o, nu, beta, r);
+    return true;
 }
 
 // genorb_blob (src/functions.cu:194-232): a = a_blob + normal sigma (> 0), the
 // blob's e, i, w, Omega, true anomaly nu_blob + normal sigma / a; for a
 // collision cloud a velocity kick of vkepp % of the orbital speed in an
-// isotropic direction.
-void orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta, Rng& g, double* r)
+// isotropic direction.  Returns false if no positive a was drawn.
+bool orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta, Rng& g, double* r)
 {
-    double a;
-    do { a = B.a + g.normal() * B.sigma; } while (a <= 0.0);
+    double a = 0.0;
+    for (int k = 0; k < kMaxDraws && !(a > 0.0); ++k) a = B.a + g.normal() * B.sigma;
+    if (!(a > 0.0)) return false;
     const double nu = B.nu + g.normal() * B.sigma / a;
     ddyn_impl::coe2rv(c.pm[0], m, a, B.ecc, B.inc, B.w, B.o, nu, beta, r);
     if (B.collcloud == 0) {
@@ -307,6 +315,7 @@ void orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta,
         r[4] += voff * st * std::sin(gam);
         r[5] += voff * ct;
     }
+    return true;
 }
 
 // getbod (src/functions.cu:235-240): the next body of the mass ladder.  The
@@ -372,8 +381,7 @@ int dust_of(const ddyn_config& c, const ddyn_optics* o, const DustBins& b, doubl
         const double bt = bin_beta(beta, n_bins, id);
         for (int k = 0; k < b.npg; ++k) {
             double r[6];
-            orbit(m, bt, g, r);
-            if (!out.add(r, N / b.npg, m, id)) return kErrInterp;
+            if (!orbit(m, bt, g, r) || !out.add(r, N / b.npg, m, id)) return kErrInterp;
         }
     }
     return 0;
@@ -603,8 +611,7 @@ int ddyn_generate(const ddyn_config* c, const ddyn_optics* o, const float* beta,
         for (int64_t l = 0; l < D.Ngrav; ++l) {
             const double m = next_body(mmax, D.C, D.eta);
             double r[6];
-            orbit_disk(*c, D, m, 0.0, g, r);
-            if (!out.add(r, 1.0, m, -1)) return kErrInterp;
+            if (!orbit_disk(*c, D, m, 0.0, g, r) || !out.add(r, 1.0, m, -1)) return kErrInterp;
         }
     }
     for (int j = 0; j < c->n_blobs; ++j) {
@@ -613,7 +620,7 @@ int ddyn_generate(const ddyn_config* c, const ddyn_optics* o, const float* beta,
         for (int64_t l = 0; l < B.Ngrav; ++l) {
             const double m = next_body(mmax, B.C, B.eta);
             double r[6];
-            orbit_blob(*c, B, m, 0.0, g, r);
+            if (!orbit_blob(*c, B, m, 0.0, g, r)) return kErrInterp;
             if (!out.add(r, 1.0, m, -1)) return kErrInterp;
         }
     }
@@ -622,13 +629,13 @@ int ddyn_generate(const ddyn_config* c, const ddyn_optics* o, const float* beta,
     for (int j = 0; j < c->n_disks; ++j) {
         const ddyn_disk& D = c->disks[j];
         const DustBins b = dust_bins(*c, D.dmmax, D.mmin, D.Nsmall);
-        auto orb = [&](double m, double bt, Rng& gg, double* r) { orbit_disk(*c, D, m, bt, gg, r); };
+        auto orb = [&](double m, double bt, Rng& gg, double* r) { return orbit_disk(*c, D, m, bt, gg, r); };
         if ((rc = dust_of(*c, o, b, D.C, D.eta, beta, n_bins, g, out, orb))) return rc;
     }
     for (int j = 0; j < c->n_blobs; ++j) {
         const ddyn_blob& B = c->blobs[j];
         const DustBins b = dust_bins(*c, B.dmmax, B.mmin, B.Nsmall);
-        auto orb = [&](double m, double bt, Rng& gg, double* r) { orbit_blob(*c, B, m, bt, gg, r); };
+        auto orb = [&](double m, double bt, Rng& gg, double* r) { return orbit_blob(*c, B, m, bt, gg, r); };
         if ((rc = dust_of(*c, o, b, B.C, B.eta, beta, n_bins, g, out, orb))) return rc;
     }
     return out.n == n_all ? 0 : kErrInterp;

@@ -158,7 +158,9 @@ int ddyn_generate_counts(const ddyn_config *cfg, int64_t *n_mass, int64_t *n_dus
  * id -1; may be NULL).  Orbits follow the reference's distributions (surface
  * density ~ r^gamma, |normal| eccentricities and inclinations, uniform angles;
  * Gaussian blobs with an optional collisional velocity kick) from this
- * library's own seeded generator: the same laws, not the reference's numbers. */
+ * library's own seeded generator: the same laws, not the reference's numbers.
+ * Returns 54 when n_all does not match the counts, or when a law's rejection
+ * draw (e < 1 for a disk, a > 0 for a blob) fails 2^20 times in a row. */
 int ddyn_generate(const ddyn_config *cfg, const ddyn_optics *o, const float *beta, int32_t n_bins, int64_t n_all,
                   double *x, double *y, double *z, double *vx, double *vy, double *vz, double *N, double *mu,
                   int32_t *id);

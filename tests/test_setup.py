@@ -293,6 +293,22 @@ def test_generation_is_seeded_and_counts_match(tmp_path):
     assert np.array_equal(st.x[:cfg.n_planets], st3.x[:cfg.n_planets])
 
 
+def test_generation_refuses_laws_it_cannot_draw(tmp_path):
+    """A blob whose semi-major axes can never be positive (a < 0, no spread)
+    and a disk whose eccentricities are almost never below 1 end the
+    generation with error 54 instead of looping."""
+    for disks, blobs in (("", "-5.0 0.1 5.0 10.0 20.0 30.0 100.0 1.0 3.67 0.1 10 0 cloud 0.0"),
+                         ("30.0 40.0 0.05 1.0e12 100.0 1.0 3.67 0.1 -1.0 10 0", "")):
+        spec, comp = tmp_path / "star.dat", tmp_path / "bb.dat"
+        OL.write_spectrum(spec, 5800.0)
+        OL.gray_table(comp, SIZES_UM, WAV_UM, n_theta=21)
+        cfg, opt = _setup(tmp_path, comp, spec, disks, blobs)
+        t = hostio.grain_tables(cfg, opt)
+        with pytest.raises(hostio.HostIOError) as e:
+            hostio.generate(cfg, opt, t["beta"])
+        assert e.value.code == 54
+
+
 def test_restart_ids_from_masses(tmp_path):
     """Grid::read_in recomputes each body's bin from its mass; beyond the grid
     (gravitating bodies) and massless bodies get -1."""
