@@ -50,11 +50,14 @@ const char* error_text(int code)
         case 36: return "system geometry needs seven values";
         case 37: return "distance must be > 0";
         case 38: return "field of view must be > 0";
+        case 39: return "spectral or optical data file missing";
         case 40: return "Teff outside the spectral library range";
         case 41: return "image wavelengths are required";
+        case 42: return "optical data not well formed";
         case 44: return "too many planets";
         case 45: return "too many disks";
         case 46: return "too many blobs";
+        case 54: return "interpolation or particle generation error";
         case 55: return "data file version differs from this version";
         case 56: return "data file nR differs from the compiled radial grid size";
         case 57: return "blob needs 'cloud' or 'collision'";

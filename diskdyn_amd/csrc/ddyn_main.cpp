@@ -258,18 +258,29 @@ int main(int argc, char** argv)
     };
     clip();
     int64_t steps = 0;
+    // The time bookkeeping is host arithmetic only, so consecutive steps of the
+    // same size with no data file between them go to the GPU as one ddyn_step
+    // call (same steps, same results, and the library can overlap or fuse them).
+    constexpr int64_t kMaxRun = 4096;
     do {
-        if (check(ctx, ddyn_step(ctx, dttmp, 1), "step")) { ddyn_destroy(ctx); return 1; }
-        ++steps;
-        run.time += dttmp;
-        if (run.time >= run.tlasts + cfg.t_sed || run.time >= cfg.t_end) { run.tlasts = run.time; dttmp = dt; }
-        if (run.time >= run.tlastf + cfg.t_fits || run.time >= cfg.t_end) { run.tlastf = run.time; dttmp = dt; }
-        if (run.time >= run.tlastd + cfg.t_write || run.time >= cfg.t_end) {
-            run.tlastd = run.time;
-            if (write(run.time)) { ddyn_destroy(ctx); return 1; }
-            dttmp = dt;
-        }
-        clip();
+        const double h = dttmp;
+        int64_t n = 0;
+        bool out = false;
+        do {
+            ++n;
+            run.time += h;
+            if (run.time >= run.tlasts + cfg.t_sed || run.time >= cfg.t_end) { run.tlasts = run.time; dttmp = dt; }
+            if (run.time >= run.tlastf + cfg.t_fits || run.time >= cfg.t_end) { run.tlastf = run.time; dttmp = dt; }
+            if (run.time >= run.tlastd + cfg.t_write || run.time >= cfg.t_end) {
+                run.tlastd = run.time;
+                out = true;
+                dttmp = dt;
+            }
+            clip();
+        } while (!out && dttmp == h && run.time < cfg.t_end && n < kMaxRun);
+        if (check(ctx, ddyn_step(ctx, h, n), "step")) { ddyn_destroy(ctx); return 1; }
+        steps += n;
+        if (out && write(run.time)) { ddyn_destroy(ctx); return 1; }
     } while (run.time < cfg.t_end);
     if (!quiet) std::printf("ddyn: %lld steps to t = %.6f yr\n", (long long)steps, run.time);
     ddyn_destroy(ctx);
