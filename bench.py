@@ -17,8 +17,9 @@ ring, the massive swarm-pair kernel on the swarms): 20 FLOP per directed
 interaction (FMA = 2, rsqrt = 1) over its mean launch time measured with HIP
 events on the stream it runs on, against the FP32 vector peak.  The CPU
 baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample
-of whole RK4 steps.  The self-gravitating swarm (65,537 bodies) and the large
-disk (1,048,577 bodies) run too, at reduced step counts, under "also".
+of whole RK4 steps.  The self-gravitating swarm (65,537 bodies), the large
+disk (1,048,577 bodies) and the debris ring with 1,048,576 grains run too, at
+reduced step counts, under "also".
 
 Prints one JSON line.
 """
@@ -453,7 +454,7 @@ def main():
                     help="steps for the FP64 comparison runs (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-also", action="store_true",
-                    help="skip the swarm and large-disk lines that the debris-ring run adds under 'also'")
+                    help="skip the swarm, large-disk and 1,048,576-grain lines that the debris-ring run adds under 'also'")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
     ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
@@ -498,16 +499,18 @@ def main():
     out.update(m)
     if (args.workload == "cfg3" and not args.no_also and args.steps is None and not args.in_order
             and args.n_dust is None):
-        # the swarm and the disk at reduced step counts: their own throughput,
-        # roofline and FP32-force speed-up (no CPU baseline)
+        # the swarm and the disk at reduced step counts, and the debris ring with
+        # four times the grains: their own throughput, roofline and FP32-force
+        # speed-up (no CPU baseline)
         also = {}
-        for name, st32, st64 in (("cfg2", 200, 20), ("cfg5", 2, 1)):
-            a = measure(name, st32, st64, 1)
-            also[name] = {k: a[k] for k in ("value", "ms_per_step", "steps_per_s", "steps", "config",
-                                            "force_rel_err", "speedup_fp32_vs_fp64", "roofline", "state_update")}
-            also[name]["fp64"] = {k: a["fp64"][k] for k in ("value", "ms_per_step", "steps", "kernel_frac_fp64_peak")}
-            also[name]["fp64_rsq"] = {k: a["fp64_rsq"][k] for k in ("value", "ms_per_step", "steps",
-                                                                     "speedup_fp32_vs_fp64_rsq")}
+        for key, name, st32, st64, nd in (("cfg2", "cfg2", 200, 20, None), ("cfg5", "cfg5", 2, 1, None),
+                                          ("cfg3_1048576_grains", "cfg3", 40, 4, 1048576)):
+            a = measure(name, st32, st64, 1, n_dust=nd)
+            also[key] = {k: a[k] for k in ("value", "ms_per_step", "steps_per_s", "steps", "config",
+                                           "force_rel_err", "speedup_fp32_vs_fp64", "roofline", "state_update")}
+            also[key]["fp64"] = {k: a["fp64"][k] for k in ("value", "ms_per_step", "steps", "kernel_frac_fp64_peak")}
+            also[key]["fp64_rsq"] = {k: a["fp64_rsq"][k] for k in ("value", "ms_per_step", "steps",
+                                                                    "speedup_fp32_vs_fp64_rsq")}
         out["also"] = also
     print(json.dumps(out), flush=True)
     return 0
