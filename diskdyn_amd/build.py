@@ -1,6 +1,7 @@
 """In-tree build of the native parts.
 
-* libddyn_hip.so  HIP kernels + C ABI (include/ddyn.h), hipcc for gfx950 with
+* libddyn_hip.so  HIP kernels + C ABI (include/ddyn.h): ddyn_hip.hip (state,
+  steps, adjoint) and ddyn_image.hip (images and SEDs), hipcc for gfx950 with
   -O3, C++17, -ffp-contract=off (every fused multiply-add in the kernels is
   written out explicitly, so rounding is fixed) and the packed-FP32 target
   feature switched off.
@@ -35,7 +36,8 @@ def _run(cmd, what):
 
 def build_hip(verbose=False):
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "ddyn_hip.hip"), "-o", tmp], "hipcc")
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "ddyn_hip.hip"), os.path.join(CSRC, "ddyn_image.hip"),
+                                "-o", tmp], "hipcc")
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

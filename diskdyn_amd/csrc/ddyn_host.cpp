@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -640,6 +641,137 @@ int ddyn_read_datafile(const char* path, const char* version, ddyn_config* c, dd
     if (rc) return rc;
     if (x && k != run->n_all) return 17;
     return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Images (FITS) and SEDs
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double kCspeed = 299792458.0;
+
+// FITS header cards (the FITS standard, NASA/IAU 4.0: 80-character cards in
+// 2880-byte blocks; a fixed-format value right-justified in columns 11-30, a
+// string value quoted from column 11, the comment after " / ").
+void card(std::string& h, const char* key, const std::string& value, bool quoted, const char* comment)
+{
+    char buf[96];
+    if (quoted) {
+        std::string q = "'" + value;
+        while (q.size() < 9) q += ' ';   // at least eight characters inside the quotes
+        q += "'";
+        std::snprintf(buf, sizeof buf, "%-8.8s= %-20s / %s", key, q.c_str(), comment);
+    } else {
+        std::snprintf(buf, sizeof buf, "%-8.8s= %20s / %s", key, value.c_str(), comment);
+    }
+    std::string c(buf);
+    c.resize(80, ' ');
+    h += c;
+}
+
+void text_card(std::string& h, const char* key, const char* text)
+{
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "%-8.8s%s", key, text);
+    std::string c(buf);
+    c.resize(80, ' ');
+    h += c;
+}
+
+// A real value with `digits` significant digits, always with a decimal point
+// or an exponent (a FITS real, not an integer).
+std::string real(double v, int digits)
+{
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%.*G", digits, v);
+    std::string s(buf);
+    if (s.find_first_of(".EN") == std::string::npos) s += ".";
+    return s;
+}
+
+void copy_name(char* name, int32_t name_len, const char* s)
+{
+    if (name && name_len > 0) {
+        std::strncpy(name, s, (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ddyn_write_image(const char* version, const ddyn_config* c, double time, double wvl, const double* F,
+                     char* name, int32_t name_len)
+{
+    if (!c || !F || c->xsize < 1 || c->ysize < 1) return 39;
+    char fname[DDYN_STR + 64];
+    std::snprintf(fname, sizeof fname, "%s_%.2f_%.3f.fits", c->name_stub, wvl * 1e6, time);
+    copy_name(name, name_len, fname);
+    const int64_t nx = c->xsize, ny = c->ysize;
+    // row y of the FITS array holds pixels (0..nx-1, y); the total in the
+    // reference's order (x outer, y inner, src/fits.cu:52-58)
+    double Ptot = 0.0;
+    for (int64_t x = 0; x < nx; ++x)
+        for (int64_t y = 0; y < ny; ++y) Ptot += F[ny * x + y];
+    const int size = c->xsize >= c->ysize ? c->xsize : c->ysize;
+    const float pxscale = (float)(c->fov / c->dist * kPc / size);
+    const float lambda = (float)(wvl * 1e6);
+    std::string h;
+    card(h, "SIMPLE", "T", false, "file does conform to FITS standard");
+    card(h, "BITPIX", "-64", false, "number of bits per data pixel");
+    card(h, "NAXIS", "2", false, "number of data axes");
+    card(h, "NAXIS1", std::to_string(nx), false, "length of data axis 1");
+    card(h, "NAXIS2", std::to_string(ny), false, "length of data axis 2");
+    card(h, "EXTEND", "T", false, "FITS dataset may contain extensions");
+    card(h, "WAVELENG", real(lambda, 7), false, "[micron] Wavelength of dataset");
+    card(h, "FLUX", real(Ptot, 15), false, "[Jy] Total flux detected");
+    card(h, "PIXELSCA", real(pxscale, 7), false, "[''/px] pixel scale of instrument setup");
+    card(h, "VERSION", version ? version : "", true, "DiskDyn version");
+    text_card(h, "HISTORY", "Created with DiskDyn");
+    {
+        char date[32];
+        const std::time_t now = std::time(nullptr);
+        std::tm tmv{};
+        gmtime_r(&now, &tmv);
+        std::strftime(date, sizeof date, "%Y-%m-%dT%H:%M:%S", &tmv);
+        card(h, "DATE", date, true, "file creation date (YYYY-MM-DDThh:mm:ss UT)");
+    }
+    text_card(h, "END", "");
+    h.resize((h.size() + 2879) / 2880 * 2880, ' ');
+    // data: FP64 big-endian, NAXIS1 (x) fastest, zero-padded to whole blocks
+    std::vector<unsigned char> data((size_t)(nx * ny * 8 + 2879) / 2880 * 2880, 0);
+    for (int64_t y = 0; y < ny; ++y)
+        for (int64_t x = 0; x < nx; ++x) {
+            uint64_t u;
+            std::memcpy(&u, &F[ny * x + y], 8);
+            unsigned char* d = &data[(size_t)(y * nx + x) * 8];
+            for (int b = 0; b < 8; ++b) d[b] = (unsigned char)(u >> (56 - 8 * b));
+        }
+    std::remove(fname);
+    FILE* fw = std::fopen(fname, "wb");
+    if (!fw) return 39;
+    const bool ok = std::fwrite(h.data(), 1, h.size(), fw) == h.size() &&
+                    std::fwrite(data.data(), 1, data.size(), fw) == data.size();
+    return (std::fclose(fw) == 0 && ok) ? 0 : 39;
+}
+
+int ddyn_write_sed(const char* version, const ddyn_config* c, double time, int32_t n_wav, const double* w,
+                   const double* in, const double* Fexc, char* name, int32_t name_len)
+{
+    if (!c || n_wav < 0 || (n_wav > 0 && (!w || !in || !Fexc))) return 39;
+    char fname[DDYN_STR + 64];
+    std::snprintf(fname, sizeof fname, "%s_SED_%.2f.dat", c->name_stub, time);
+    copy_name(name, name_len, fname);
+    FILE* fw = std::fopen(fname, "wb");
+    if (!fw) return 39;
+    std::fprintf(fw, "#Calculated with DiskDyn Version %s\n", version ? version : "");
+    std::fprintf(fw, "#Wav[mu]\tFlux_star[Jy]\tFlux_exc[Jy]\n");
+    for (int32_t j = 0; j < n_wav; ++j)
+        std::fprintf(fw, "%.3f %.6e %.6e\n", w[j] * 1e6, in[j] * 1e26 * w[j] * w[j] / kCspeed, Fexc[j]);
+    return std::fclose(fw) == 0 ? 0 : 39;
 }
 
 }  // extern "C"

@@ -33,6 +33,13 @@ struct ddyn_optics {
     std::vector<double> spec_w, spec_F;      // stellar spectrum: wavelength [m], flux at the observer
 };
 
+// The dust grid's optical tables for images and SEDs (ddyn_grid_optics_make).
+struct ddyn_grid_optics {
+    int n_bins = 0, n_wav = 0, n_theta = 0;
+    std::vector<double> w, in, size, Qabs, Qsca, Qpf, Td;   // as ddyn_optics_tables describes them
+    double Rmin = 0.0, Rmax = 0.0;                          // radial grid of Td [AU]
+};
+
 namespace {
 
 // Constants of src/ddyn.cuh:16-38.
@@ -55,6 +62,8 @@ constexpr int kMaxBins = 2700;           // size bins the device tables hold (ns
 constexpr int kErrFile = 39;       // a file could not be opened
 constexpr int kErrOptics = 42;     // optical data not well formed
 constexpr int kErrInterp = 54;     // interpolation / grid error
+constexpr int kErrWav = 43;        // an image wavelength outside the optical data
+constexpr int kNR = 100;           // radii of the dust temperature table (nR, src/ddyn.cuh:34)
 
 bool read_lines(const char* path, std::vector<std::string>& lines)
 {
@@ -200,6 +209,83 @@ double loglog(double x, double x0, double x1, double y0, double y1)
 double planck(double T, double wvl)
 {
     return (2 * kPlanck * kC * kC) / (std::pow(wvl, 5) * (std::exp(kPlanck * kC / (wvl * kBoltz * T)) - 1));
+}
+
+// The optical data on the dust grid's sizes and the stellar flux on the optical
+// wavelengths (interpolate_dust_to_grid and interpolate_stellar_to_grid,
+// src/functions.cu:243-327): sizes a_i = amin exp(i f_a), efficiencies and the
+// phase function log-log in size between the table's sizes; the spectrum
+// log-log in wavelength, zero below its first wavelength and a Rayleigh-Jeans
+// tail beyond its last.
+struct GridTables {
+    std::vector<double> a, Qa, Qs, Qp, in;   // [bin], [bin][wav], [bin][wav], [bin][theta][wav], [wav]
+};
+
+int grid_tables(const ddyn_config* c, const ddyn_optics* o, GridTables& g)
+{
+    const int n = c->n_grid;
+    if (n < 2 || n > kMaxBins || !(c->amax > c->amin) || !(c->amin > 0.0)) return kErrInterp;
+    const size_t nw = (size_t)o->n_wav, nt = (size_t)o->n_theta;
+    const double fa = std::log(c->amax / c->amin) / ((double)n - 1.0);
+    g.a.assign((size_t)n, 0.0);
+    g.Qa.assign((size_t)n * nw, 0.0);
+    g.Qs.assign((size_t)n * nw, 0.0);
+    g.Qp.assign((size_t)n * nt * nw, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double ai = c->amin * std::exp((double)i * fa);
+        if (i == n - 1 && (ai - o->dmax) > 1e-12) return kErrInterp;
+        if (i == n - 1 && (ai - o->dmax) < 1e-12 && (ai - o->dmax) > 0) ai = o->dmax;
+        g.a[(size_t)i] = ai;
+        const size_t ip = upper_index(o->s, ai), im = ip - 1;
+        const double s0 = o->s[im], s1 = o->s[ip];
+        for (size_t j = 0; j < nw; ++j) {
+            g.Qa[i * nw + j] = loglog(ai, s0, s1, o->Qabs[im * nw + j], o->Qabs[ip * nw + j]);
+            g.Qs[i * nw + j] = loglog(ai, s0, s1, o->Qsca[im * nw + j], o->Qsca[ip * nw + j]);
+            for (size_t k = 0; k < nt; ++k)
+                g.Qp[nw * (nt * i + k) + j] =
+                    loglog(ai, s0, s1, o->Qpf[nw * (nt * im + k) + j], o->Qpf[nw * (nt * ip + k) + j]);
+        }
+    }
+    g.in.assign(nw, 0.0);
+    const size_t nsp = o->spec_w.size();
+    for (size_t j = 0; j < nw; ++j) {
+        const double wj = o->w[j];
+        if (wj < o->spec_w[0]) {
+            g.in[j] = 0.0;
+        } else if (wj > o->spec_w[nsp - 1]) {
+            g.in[j] = o->spec_F[nsp - 1] * std::pow(o->spec_w[nsp - 1] / wj, 4.0);   // Rayleigh-Jeans tail
+        } else {
+            size_t ip = 0;
+            while (ip < nsp - 1 && o->spec_w[ip] <= wj) ++ip;
+            if (ip == 0) ip = 1;
+            const size_t im = ip - 1;
+            const double F0 = o->spec_F[im], F1 = o->spec_F[ip];
+            g.in[j] = (F0 != 0 && F1 != 0) ? loglog(wj, o->spec_w[im], o->spec_w[ip], F0, F1) : 0.0;
+        }
+    }
+    return 0;
+}
+
+// The sublimation radius [m] (calculate_dust_temp, src/grid.cu:766-783): the
+// smallest over the bins of the distance at which a grain reaches T_sub,
+// 0.5 sqrt(absorbed / emitted), starting from Rmax_m.
+double sublimation_radius(const ddyn_config* c, const ddyn_optics* o, const GridTables& g, double Rmax_m)
+{
+    const size_t nw = (size_t)o->n_wav;
+    double Rmin = Rmax_m;
+    for (int k = 0; k < c->n_grid; ++k) {
+        double sk = 0.0, gf = 0.0;
+        for (size_t i = 1; i < nw; ++i) {
+            sk += 0.5 * (o->w[i] - o->w[i - 1]) * (g.Qa[k * nw + i - 1] * g.in[i - 1] + g.Qa[k * nw + i] * g.in[i]) /
+                  kPI;
+            gf += 0.5 * (o->w[i] - o->w[i - 1]) *
+                  (g.Qa[k * nw + i - 1] * planck(kTsub, o->w[i - 1]) + g.Qa[k * nw + i] * planck(kTsub, o->w[i]));
+        }
+        sk *= std::pow(c->dist, 2.0);
+        const double Rt = 0.5 * std::sqrt(sk / gf);
+        if (Rt < Rmin) Rmin = Rt;
+    }
+    return Rmin;
 }
 
 // The seeded random stream of particle generation: Mitchell and Moore's
@@ -486,46 +572,14 @@ int ddyn_grain_tables(const ddyn_config* c, const ddyn_optics* o, int32_t n_max,
 {
     if (!c || !o) return kErrOptics;
     const int n = c->n_grid;
-    if (n < 2 || n > kMaxBins || n > n_max || !(c->amax > c->amin) || !(c->amin > 0.0)) return kErrInterp;
+    if (n > n_max) return kErrInterp;
+    GridTables g;
+    if (const int rc = grid_tables(c, o, g)) return rc;
     const size_t nw = (size_t)o->n_wav, nt = (size_t)o->n_theta;
-    // interpolate_dust_to_grid: sizes of the grid, optical data log-log in size
-    const double fa = std::log(c->amax / c->amin) / ((double)n - 1.0);
-    std::vector<double> a(n), Qa((size_t)n * nw), Qs((size_t)n * nw), Qp((size_t)n * nt * nw);
-    for (int i = 0; i < n; ++i) {
-        a[i] = c->amin * std::exp((double)i * fa);
-        if (i == n - 1 && (a[i] - o->dmax) > 1e-12) return kErrInterp;
-        if (i == n - 1 && (a[i] - o->dmax) < 1e-12 && (a[i] - o->dmax) > 0) a[i] = o->dmax;
-        const size_t ip = upper_index(o->s, a[i]), im = ip - 1;
-        const double s0 = o->s[im], s1 = o->s[ip];
-        for (size_t j = 0; j < nw; ++j) {
-            Qa[i * nw + j] = loglog(a[i], s0, s1, o->Qabs[im * nw + j], o->Qabs[ip * nw + j]);
-            Qs[i * nw + j] = loglog(a[i], s0, s1, o->Qsca[im * nw + j], o->Qsca[ip * nw + j]);
-            for (size_t k = 0; k < nt; ++k)
-                Qp[nw * (nt * i + k) + j] =
-                    loglog(a[i], s0, s1, o->Qpf[nw * (nt * im + k) + j], o->Qpf[nw * (nt * ip + k) + j]);
-        }
-    }
-    // interpolate_stellar_to_grid: the stellar flux at the optical wavelengths
-    std::vector<double> in(nw);
-    const size_t nsp = o->spec_w.size();
-    for (size_t j = 0; j < nw; ++j) {
-        const double wj = o->w[j];
-        if (wj < o->spec_w[0]) {
-            in[j] = 0.0;
-        } else if (wj > o->spec_w[nsp - 1]) {
-            in[j] = o->spec_F[nsp - 1] * std::pow(o->spec_w[nsp - 1] / wj, 4.0);   // Rayleigh-Jeans tail
-        } else {
-            size_t ip = 0;
-            while (ip < nsp - 1 && o->spec_w[ip] <= wj) ++ip;
-            if (ip == 0) ip = 1;
-            const size_t im = ip - 1;
-            const double F0 = o->spec_F[im], F1 = o->spec_F[ip];
-            in[j] = (F0 != 0 && F1 != 0) ? loglog(wj, o->spec_w[im], o->spec_w[ip], F0, F1) : 0.0;
-        }
-    }
+    const std::vector<double>& a = g.a;
     // calculate_beta: radiation pressure efficiency averaged over the spectrum
     double star = 0.0;
-    for (size_t j = 0; j + 1 < nw; ++j) star += 0.5 * (o->w[j + 1] - o->w[j]) * (in[j + 1] + in[j]);
+    for (size_t j = 0; j + 1 < nw; ++j) star += 0.5 * (o->w[j + 1] - o->w[j]) * (g.in[j + 1] + g.in[j]);
     const double L = star * 4.0 * kPI * std::pow(c->dist, 2.0) / kLsunW;
     const double m0 = c->pm[0];
     const double swconst = 3.0 * c->mloss * kCd * kMSun * kYear * c->vsw / 32.0 / kPI / m0 / c->rho / kAU / kAU / kAU;
@@ -534,16 +588,16 @@ int ddyn_grain_tables(const ddyn_config* c, const ddyn_optics* o, int32_t n_max,
         double b = 0.0;
         if (c->radQ == 1 && star > 0.0) {
             for (size_t j = 0; j < nw; ++j) {
-                double g = 0.0, S = 0.0, cm = -1.0;
+                double gq = 0.0, S = 0.0, cm = -1.0;
                 for (size_t k = 1; k < nt; ++k) {
                     const double cp = (double)k * 2.0 / ((double)nt - 1.0) - 1.0;
-                    const double p0 = Qp[nw * (nt * i + k - 1) + j], p1 = Qp[nw * (nt * i + k) + j];
+                    const double p0 = g.Qp[nw * (nt * i + k - 1) + j], p1 = g.Qp[nw * (nt * i + k) + j];
                     S += 0.5 * (p0 + p1) * (cp - cm);
-                    g += 0.5 * (cm * p0 + cp * p1) * (cp - cm);
+                    gq += 0.5 * (cm * p0 + cp * p1) * (cp - cm);
                     cm = cp;
                 }
-                g = S != 0.0 ? g / S : 0.0;
-                Qpr[j] = (Qa[i * nw + j] + Qs[i * nw + j] * (1.0 - g)) * in[j];
+                gq = S != 0.0 ? gq / S : 0.0;
+                Qpr[j] = (g.Qa[i * nw + j] + g.Qs[i * nw + j] * (1.0 - gq)) * g.in[j];
             }
             double Qsum = 0.0;
             for (size_t j = 0; j + 1 < nw; ++j) Qsum += 0.5 * (o->w[j + 1] - o->w[j]) * (Qpr[j + 1] + Qpr[j]);
@@ -554,22 +608,121 @@ int ddyn_grain_tables(const ddyn_config* c, const ddyn_optics* o, int32_t n_max,
         if (q) q[i] = (float)(c->kqe * a[i] * 1e9);
         if (size) size[i] = a[i];
     }
-    // calculate_dust_temp: the sublimation radius of the most easily heated bin
-    const double Rmax = ddyn_outer_radius(c) * kAU;
-    double Rmin = Rmax;
-    for (int k = 0; k < n; ++k) {
-        double sk = 0.0, gf = 0.0;
-        for (size_t i = 1; i < nw; ++i) {
-            sk += 0.5 * (o->w[i] - o->w[i - 1]) * (Qa[k * nw + i - 1] * in[i - 1] + Qa[k * nw + i] * in[i]) / kPI;
-            gf += 0.5 * (o->w[i] - o->w[i - 1]) *
-                  (Qa[k * nw + i - 1] * planck(kTsub, o->w[i - 1]) + Qa[k * nw + i] * planck(kTsub, o->w[i]));
-        }
-        sk *= std::pow(c->dist, 2.0);
-        const double Rt = 0.5 * std::sqrt(sk / gf);
-        if (Rt < Rmin) Rmin = Rt;
-    }
-    if (Rmin_au) *Rmin_au = Rmin / kAU;
+    if (Rmin_au) *Rmin_au = sublimation_radius(c, o, g, ddyn_outer_radius(c) * kAU) / kAU;
     if (lum) *lum = L;
+    return 0;
+}
+
+int ddyn_grid_optics_make(const ddyn_config* c, const ddyn_optics* o, ddyn_grid_optics** out)
+{
+    if (!out) return kErrOptics;
+    *out = nullptr;
+    if (!c || !o) return kErrOptics;
+    GridTables g;
+    if (const int rc = grid_tables(c, o, g)) return rc;
+    const int n = c->n_grid;
+    const size_t nw = (size_t)o->n_wav;
+    ddyn_grid_optics* t = new ddyn_grid_optics();
+    t->n_bins = n;
+    t->n_wav = o->n_wav;
+    t->n_theta = o->n_theta;
+    t->w = o->w;
+    t->in = g.in;
+    t->size = g.a;
+    t->Qabs = g.Qa;
+    t->Qsca = g.Qs;
+    t->Qpf = g.Qp;
+    // calculate_dust_temp (src/grid.cu:754-834), radii in m until the end:
+    // R_max twice the outer radius, R_min the sublimation radius of the most
+    // easily heated bin; per bin the equilibrium distance of every temperature
+    // from 2 T_sub down to 4 K in steps of 2 K, then the temperature on kNR
+    // radii log-spaced from R_min to R_max, log-log between those points.
+    const double Rmax = ddyn_outer_radius(c) * kAU;
+    const double Rmin = sublimation_radius(c, o, g, Rmax);
+    const int nT = (int)std::floor((2.0 * kTsub - 3.0) * 0.5) + 1;
+    std::vector<double> star((size_t)n), Tarr((size_t)nT), Rtab((size_t)n * nT), B(nw);
+    for (int k = 0; k < n; ++k) {
+        const double* Qa = g.Qa.data() + (size_t)k * nw;
+        double sk = 0.0;
+        for (size_t i = 1; i < nw; ++i)
+            sk += 0.5 * (o->w[i] - o->w[i - 1]) * (Qa[i - 1] * g.in[i - 1] + Qa[i] * g.in[i]) / kPI;
+        star[(size_t)k] = sk * std::pow(c->dist, 2.0);
+    }
+    // the Planck function of each temperature once for every bin (the same
+    // values and sums as bin by bin)
+    int j = 0;
+    for (double T = 2.0 * kTsub; T >= 3.0; T -= 2.0) {
+        for (size_t i = 0; i < nw; ++i) B[i] = planck(T, o->w[i]);
+        for (int k = 0; k < n; ++k) {
+            const double* Qa = g.Qa.data() + (size_t)k * nw;
+            double gf = 0.0;
+            for (size_t i = 1; i < nw; ++i) gf += 0.5 * (o->w[i] - o->w[i - 1]) * (Qa[i - 1] * B[i - 1] + Qa[i] * B[i]);
+            Rtab[(size_t)k * nT + j] = 0.5 * std::sqrt(star[(size_t)k] / gf);
+        }
+        Tarr[(size_t)j] = T;
+        ++j;
+    }
+    t->Td.assign((size_t)n * kNR, 0.0);
+    for (int k = 0; k < n; ++k) {
+        const double* Rarr = Rtab.data() + (size_t)k * nT;
+        // The reference searches upwards from the previous radius's interval
+        // and indexes outside the table when a radius lies inside the hottest
+        // (4000 K) distance or beyond the coldest; here the first or last
+        // interval extrapolates instead.
+        int ip = 0;
+        for (int i = 0; i < kNR; ++i) {
+            const double R = Rmin * std::exp(std::log(Rmax / Rmin) * ((double)i) / ((double)kNR - 1.0));
+            while (ip < nT - 1 && Rarr[ip] <= R) ++ip;
+            if (ip == 0) ip = 1;
+            const int im = ip - 1;
+            t->Td[(size_t)k * kNR + i] =
+                std::exp(std::log(Tarr[(size_t)im]) + std::log(Tarr[(size_t)ip] / Tarr[(size_t)im]) *
+                                                          std::log(R / Rarr[im]) / std::log(Rarr[ip] / Rarr[im]));
+            --ip;
+        }
+    }
+    t->Rmin = Rmin / kAU;
+    t->Rmax = Rmax / kAU;
+    *out = t;
+    return 0;
+}
+
+void ddyn_grid_optics_free(ddyn_grid_optics* t) { delete t; }
+
+int ddyn_grid_optics_tables(const ddyn_grid_optics* t, ddyn_optics_tables* v)
+{
+    if (!t || !v) return kErrOptics;
+    v->n_bins = t->n_bins;
+    v->n_wav = t->n_wav;
+    v->n_theta = t->n_theta;
+    v->n_r = kNR;
+    v->w = t->w.data();
+    v->in = t->in.data();
+    v->size = t->size.data();
+    v->Qabs = t->Qabs.data();
+    v->Qsca = t->Qsca.data();
+    v->Qpfunc = t->Qpf.data();
+    v->Td = t->Td.data();
+    v->Rmin = t->Rmin;
+    v->Rmax = t->Rmax;
+    return 0;
+}
+
+int ddyn_image_wavelengths(const ddyn_config* c, const ddyn_grid_optics* t, int32_t* wj)
+{
+    if (!c || !t || (c->nwavs > 0 && !wj)) return kErrOptics;
+    const int nw = t->n_wav;
+    for (int j = 0; j < c->nwavs; ++j) {
+        const double v = c->wavs[j];
+        if (v < t->w[0] || v > t->w[(size_t)nw - 1]) return kErrWav;
+        // the first wavelength at or above the requested one, at most the
+        // second last, and the image is made one below it
+        // (src/functions.cu:329-335; where the reference would index -1, for a
+        // request equal to the first wavelength, the first is used)
+        int i = 0;
+        while (t->w[(size_t)i] < v && i < nw - 2) ++i;
+        wj[j] = i > 0 ? i - 1 : 0;
+    }
     return 0;
 }
 

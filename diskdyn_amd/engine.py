@@ -9,6 +9,7 @@ import os
 
 import numpy as np
 
+from .imaging import COpticsTables, CView, OpticsTables, View
 from .system import DiskState, Params
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -54,6 +55,9 @@ SIGNATURES = {
     "ddyn_read_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _I64, _D]),
     "ddyn_pair_plan": (ctypes.c_int, [_VP, _I32, _I32]),
     "ddyn_eval_rsqrt_fp64": (ctypes.c_int, [_VP, ctypes.c_int64, _D, _D]),
+    "ddyn_upload_optics": (ctypes.c_int, [_VP, ctypes.POINTER(COpticsTables)]),
+    "ddyn_calc_flux": (ctypes.c_int, [_VP, ctypes.POINTER(CView), ctypes.c_int32, _D]),
+    "ddyn_calc_sed": (ctypes.c_int, [_VP, ctypes.POINTER(CView), _D]),
 }
 
 _lib = None
@@ -247,6 +251,28 @@ class Engine:
         self._check(self.lib.ddyn_pair_plan(self.ctx, ctypes.byref(ns), ctypes.byref(ln)), "ddyn_pair_plan")
         return ns.value, ln.value
 
+
+    def upload_optics(self, t: OpticsTables):
+        """The dust grid's optical tables for images and SEDs."""
+        c = t.c_struct()
+        self._check(self.lib.ddyn_upload_optics(self.ctx, ctypes.byref(c)), "ddyn_upload_optics")
+        self.n_wav = t.n_wav
+
+    def calc_flux(self, view: View, wj):
+        """Image [Jy per pixel] of the current state at wavelength index wj,
+        in the layout pixel (x, y) at F[ysize * x + y]."""
+        F = np.empty(int(view.xsize) * int(view.ysize))
+        v = view.c_struct()
+        self._check(self.lib.ddyn_calc_flux(self.ctx, ctypes.byref(v), int(wj), _dp(F)), "ddyn_calc_flux")
+        return F
+
+    def calc_sed(self, view: View):
+        """Grain flux [Jy] at every wavelength of the uploaded tables."""
+        out = np.empty(getattr(self, "n_wav", 0))
+        v = view.c_struct()
+        self._check(self.lib.ddyn_calc_sed(self.ctx, ctypes.byref(v), _dp(out) if len(out) else None),
+                    "ddyn_calc_sed")
+        return out
 
     def rsqrt_fp64(self, x):
         """The library's device FP64 reciprocal square root of every x > 0."""

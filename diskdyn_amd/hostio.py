@@ -5,6 +5,7 @@ import os
 
 import numpy as np
 
+from .imaging import COpticsTables, OpticsTables
 from .system import DiskState, Params, make_state
 from .units import AU, YEAR
 
@@ -101,6 +102,20 @@ def lib():
             [_PD] * 8 + [_PI]
         L.ddyn_restart_ids.restype = ctypes.c_int
         L.ddyn_restart_ids.argtypes = [ctypes.POINTER(Config), ctypes.c_int64, _PD, _PI]
+        L.ddyn_grid_optics_make.restype = ctypes.c_int
+        L.ddyn_grid_optics_make.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.ddyn_grid_optics_free.restype = None
+        L.ddyn_grid_optics_free.argtypes = [ctypes.c_void_p]
+        L.ddyn_grid_optics_tables.restype = ctypes.c_int
+        L.ddyn_grid_optics_tables.argtypes = [ctypes.c_void_p, ctypes.POINTER(COpticsTables)]
+        L.ddyn_image_wavelengths.restype = ctypes.c_int
+        L.ddyn_image_wavelengths.argtypes = [ctypes.POINTER(Config), ctypes.c_void_p, _PI]
+        L.ddyn_write_image.restype = ctypes.c_int
+        L.ddyn_write_image.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config), ctypes.c_double, ctypes.c_double,
+                                       _PD, ctypes.c_char_p, ctypes.c_int32]
+        L.ddyn_write_sed.restype = ctypes.c_int
+        L.ddyn_write_sed.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config), ctypes.c_double, ctypes.c_int32,
+                                     _PD, _PD, _PD, ctypes.c_char_p, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -248,3 +263,50 @@ def restart_ids(cfg, st):
     ids = np.zeros(st.n_all, dtype=np.int32)
     lib().ddyn_restart_ids(ctypes.byref(cfg), st.n_all, _p(np.ascontiguousarray(st.mu)), _p(ids, _PI))
     return ids
+
+
+def grid_optics(cfg, optics):
+    """The dust grid's optical tables (ddyn_grid_optics_make) as OpticsTables,
+    with the image wavelength indices of the run (ddyn_image_wavelengths)."""
+    L = lib()
+    ptr = ctypes.c_void_p()
+    rc = L.ddyn_grid_optics_make(ctypes.byref(cfg), optics.ptr, ctypes.byref(ptr))
+    if rc:
+        raise HostIOError(rc, "grid_optics_make")
+    try:
+        v = COpticsTables()
+        L.ddyn_grid_optics_tables(ptr, ctypes.byref(v))
+        nb, nw, nt, nr = v.n_bins, v.n_wav, v.n_theta, v.n_r
+        arr = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy()
+        t = OpticsTables(w=arr(v.w, nw), in_=arr(v.in_, nw), size=arr(v.size, nb),
+                         Qabs=arr(v.Qabs, nb * nw).reshape(nb, nw), Qsca=arr(v.Qsca, nb * nw).reshape(nb, nw),
+                         Qpfunc=arr(v.Qpfunc, nb * nt * nw).reshape(nb, nt, nw), Td=arr(v.Td, nb * nr).reshape(nb, nr),
+                         Rmin=v.Rmin, Rmax=v.Rmax)
+        wj = np.zeros(max(cfg.nwavs, 1), dtype=np.int32)
+        rc = L.ddyn_image_wavelengths(ctypes.byref(cfg), ptr, _p(wj, _PI))
+        if rc:
+            raise HostIOError(rc, "image_wavelengths")
+        return t, wj[: cfg.nwavs]
+    finally:
+        L.ddyn_grid_optics_free(ptr)
+
+
+def write_image(cfg, time, wvl, F, version=VERSION):
+    """FITS image <stub>_<wvl um>_<time>.fits; returns the file name."""
+    name = ctypes.create_string_buffer(STR + 64)
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    rc = lib().ddyn_write_image(version.encode(), ctypes.byref(cfg), float(time), float(wvl), _p(F), name, len(name))
+    if rc:
+        raise HostIOError(rc, "write_image")
+    return name.value.decode()
+
+
+def write_sed(cfg, time, tables, Fexc, version=VERSION):
+    """SED file <stub>_SED_<time>.dat; returns the file name."""
+    name = ctypes.create_string_buffer(STR + 64)
+    Fexc = np.ascontiguousarray(Fexc, dtype=np.float64)
+    rc = lib().ddyn_write_sed(version.encode(), ctypes.byref(cfg), float(time), tables.n_wav, _p(tables.w),
+                              _p(tables.in_), _p(Fexc), name, len(name))
+    if rc:
+        raise HostIOError(rc, "write_sed")
+    return name.value.decode()

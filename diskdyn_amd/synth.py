@@ -16,7 +16,8 @@ import numpy as np
 
 from .orbits import coe2rv, kepler_true_anomaly, recalc_com
 from .system import FP32_FORCE, FP64, Params, Workload, make_state
-from .units import GRAV, LIGHT_AU, kms_to_au_per_yr
+from .imaging import OpticsTables, View
+from .units import GRAV, LIGHT_AU, PC, kms_to_au_per_yr
 
 DEG = math.pi / 180.0
 VSW_DEFAULT_KMS = 450.0     # stellar-wind speed of the reference's example inputs [km/s]
@@ -233,3 +234,42 @@ def config5(n_swarm=1048576, seed=1005, nsteps=20, dt=0.02, precision=FP32_FORCE
 
 
 CONFIGS = {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}
+
+
+def ring_imaging(w, n_bins=64, n_wav=48, n_theta=41, n_r=100, teff=3600.0, dist_pc=10.0, xsize=256, ysize=256,
+                 fov=100.0, incl_deg=30.0, pa_deg=20.0, rot_deg=10.0, Rmin=0.1, Rmax=200.0):
+    """Optical tables and an observer for imaging the debris ring of config3
+    (which must carry its grain sizes, w.meta["s_um"]): the grains get size-bin
+    ids on n_bins log bins over [1, 1000] um; smooth synthetic optics with
+    x = 2 pi s / lambda: Qabs = x / (1 + x), Qsca = 0.5 x^2 / (1 + x^2), a
+    forward-peaked phase function exp(a mu) with a = 2 x / (1 + x); a
+    blackbody star of teff at dist_pc (radius from L = 0.1 Lsun); grain
+    temperatures 278.3 K (L/Lsun)^(1/4) (R/AU)^(-1/2) (1 + 1/x_10um)^(1/5) on
+    n_r radii from Rmin to Rmax.  Sets the grains' ids in w.state; returns the
+    tables and the view."""
+    st = w.state
+    s_um = np.asarray(w.meta["s_um"], dtype=np.float64)
+    smin, smax = 1.0, 1000.0
+    b = np.floor(np.log(s_um / smin) / np.log(smax / smin) * n_bins).astype(np.int64)
+    st.id[st.n_grav:] = np.clip(b, 0, n_bins - 1).astype(np.int32)
+    size_um = smin * (smax / smin) ** ((np.arange(n_bins) + 0.5) / n_bins)
+    wav = np.geomspace(0.3e-6, 1.0e-3, n_wav)
+    x = 2.0 * math.pi * size_um[:, None] * 1e-6 / wav[None, :]
+    qabs = x / (1.0 + x)
+    qsca = 0.5 * x * x / (1.0 + x * x)
+    mu = np.linspace(-1.0, 1.0, n_theta)
+    a = 2.0 * x / (1.0 + x)
+    qpf = np.exp(a[:, None, :] * mu[None, :, None])
+    h, c, k = 6.626068e-34, 299792458.0, 1.3806503e-23
+    lum = 0.1 * 3.827e26
+    sigma = 2.0 * math.pi ** 5 * k ** 4 / (15.0 * h ** 3 * c ** 2)
+    rstar = math.sqrt(lum / (4.0 * math.pi * sigma * teff ** 4))
+    dist = dist_pc * PC
+    influx = math.pi * 2 * h * c * c / (wav ** 5 * (np.exp(h * c / (wav * k * teff)) - 1.0)) * (rstar / dist) ** 2
+    R = Rmin * (Rmax / Rmin) ** (np.arange(n_r) / (n_r - 1.0))
+    x10 = 2.0 * math.pi * size_um / 10.0
+    Td = 278.3 * 0.1 ** 0.25 * R[None, :] ** -0.5 * (1.0 + 1.0 / x10[:, None]) ** 0.2
+    t = OpticsTables(w=wav, in_=influx, size=size_um * 1e-6, Qabs=qabs, Qsca=qsca, Qpfunc=qpf, Td=Td, Rmin=Rmin,
+                     Rmax=Rmax)
+    v = View(dist=dist, rot=rot_deg * DEG, incl=incl_deg * DEG, PA=pa_deg * DEG, fov=fov, xsize=xsize, ysize=ysize)
+    return t, v

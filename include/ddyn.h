@@ -211,6 +211,67 @@ int ddyn_read_timing(ddyn_ctx *ctx, int32_t kclass, int64_t *launches, double *t
  * and a state.  No reference counterpart (measurement support). */
 int ddyn_pair_plan(ddyn_ctx *ctx, int32_t *nslice, int32_t *slice_len);
 
+/* ---------------------------------------------------------------------------
+ * Images and SEDs of the grains' thermal emission and scattered starlight
+ * (src/gpu_functions.cu:945-1156, driven by Grid::null_flux, calc_flux,
+ * calc_flux_sed and copy_sed, src/grid.cu:12-65).
+ * ------------------------------------------------------------------------- */
+
+/* The dust grid's optical tables (the reference's Qabs, Qsca, Qpfc, Td, w, in
+ * device arrays and the size table, uploaded in sync_to_gpu,
+ * src/init.cu:595-664; ddyn_grid_optics_tables in ddyn_host.h fills one from a
+ * .param run's optical data).  Bin b is the grains' size-bin id. */
+typedef struct {
+    int32_t n_bins;        /* size bins */
+    int32_t n_wav;         /* wavelengths */
+    int32_t n_theta;       /* scattering-angle points, cos from -1 to 1 */
+    int32_t n_r;           /* radii of the temperature table (nR = 100 in the reference) */
+    const double *w;       /* n_wav wavelengths [m], increasing */
+    const double *in;      /* n_wav stellar flux at the observer [W m^-3] */
+    const double *size;    /* n_bins grain radii [m] */
+    const double *Qabs;    /* n_bins * n_wav absorption efficiencies, [bin][wav] */
+    const double *Qsca;    /* n_bins * n_wav scattering efficiencies, [bin][wav] */
+    const double *Qpfunc;  /* n_bins * n_theta * n_wav phase function, [bin][theta][wav] */
+    const double *Td;      /* n_bins * n_r grain temperature [K] on n_r radii log-spaced
+                              from Rmin to Rmax, [bin][radius] */
+    double Rmin, Rmax;     /* radial range of Td [AU]; grains within Rmin do not emit */
+} ddyn_optics_tables;
+
+/* Observer and detector (the reference's cfg->dist, longitude, inc, PA, xsize,
+ * ysize and fov, .param record 7). */
+typedef struct {
+    double dist;           /* distance [m] */
+    double rot;            /* rotation of the system about its z axis [rad] (longitude) */
+    double incl;           /* inclination [rad] */
+    double PA;             /* position angle [rad] */
+    double fov;            /* field of view [AU] across the larger image side */
+    int32_t xsize, ysize;  /* pixels */
+} ddyn_view;
+
+/* Copy the optical tables to the device (replaces their cudaMemcpy's in
+ * sync_to_gpu, src/init.cu:639-664).  n_bins <= 2700, n_wav and n_theta >= 2,
+ * n_r >= 2, n_bins * n_theta * n_wav <= 2^31, 0 < Rmin < Rmax. */
+int ddyn_upload_optics(ddyn_ctx *ctx, const ddyn_optics_tables *t);
+
+/* Image at wavelength index wj of the current state [Jy per pixel] (replaces
+ * null_flux + calc_flux and the download in write_image, src/grid.cu:12-54,
+ * src/fits.cu:21): F gets xsize * ysize values in the reference's layout,
+ * pixel (x, y) at F[ysize * x + y].  A grain with a size bin in the tables,
+ * outside Rmin and projected inside the field adds N (thermal + scattered)
+ * flux to its pixel.  Unlike the reference's atomic adds, each pixel sums its
+ * grains in grain order, so images are deterministic.  xsize * ysize <= 2^30.
+ * Synchronises. */
+int ddyn_calc_flux(ddyn_ctx *ctx, const ddyn_view *v, int32_t wj, double *F);
+
+/* SED of the grains [Jy] at every wavelength of the tables (replaces
+ * null_flux_part + calc_flux_sed + copy_sed per wavelength, src/grid.cu:21-41,
+ * 56-65, 366-373): Fexc gets n_wav totals over all grains, summed in a fixed
+ * order (deterministic).  As in the reference's SED kernel the system is
+ * turned about z with the grain's 3-D distance, not its cylindrical one
+ * (src/gpu_functions.cu:1104-1105); xsize, ysize and fov are not used.
+ * Synchronises. */
+int ddyn_calc_sed(ddyn_ctx *ctx, const ddyn_view *v, double *Fexc);
+
 /* y[i] = the library's FP64 reciprocal square root of x[i] > 0, evaluated on the
  * device (the v_rsq_f64 estimate refined by one third-order step, used by every
  * pair law of DDYN_FP64_RSQ and by the adjoint's Jacobian products).  n <= 2^30;

@@ -12,6 +12,8 @@
  *   ddyn_write_datafile  replaces Grid::writegrid    (src/grid.cu:441-673)
  *   ddyn_read_datafile   replaces read_in_model + Grid::read_in
  *                        (src/read_in.cu:736-1014, src/grid.cu:205-310)
+ *   ddyn_write_image     replaces Grid::write_image  (src/fits.cu:9-85)
+ *   ddyn_write_sed       replaces Grid::write_sed    (src/grid.cu:675-691)
  * Functions return 0 on success or the reference's error number (the index of
  * its message table, src/error.cu:5-64); ddyn_host_error_text() gives a text.
  * Units as in the reference's configdata after parsing.
@@ -19,6 +21,8 @@
 #ifndef DDYN_HOST_H
 #define DDYN_HOST_H
 #include <stdint.h>
+
+#include "ddyn.h"   /* ddyn_optics_tables */
 
 #ifdef __cplusplus
 extern "C" {
@@ -141,6 +145,48 @@ double ddyn_outer_radius(const ddyn_config *cfg);
  * n_grid entries), Rmin [AU] and the luminosity [Lsun] (may be NULL). */
 int ddyn_grain_tables(const ddyn_config *cfg, const ddyn_optics *o, int32_t n_max, float *beta, float *betasw,
                       float *q, double *size, double *Rmin_au, double *lum);
+
+/* The dust grid's optical tables for images and SEDs: the optical data
+ * interpolated to the grid's sizes and the stellar flux to its wavelengths (as
+ * ddyn_grain_tables does), and the grain temperature table of
+ * calculate_dust_temp (src/grid.cu:754-834): per bin, the distance at which a
+ * grain is in equilibrium at each temperature from 2 T_sub = 4000 K down to
+ * 4 K in steps of 2 K, 0.5 sqrt(absorbed / emitted) with trapezoid sums over
+ * the wavelengths; then the temperature on 100 radii log-spaced from the
+ * sublimation radius to twice the outer radius, log-log between those points.
+ * Where a radius lies inside the 4000 K distance or beyond the 4 K one (the
+ * reference then reads outside its table) the nearest interval extrapolates.
+ * Needs the dust grid (ddyn_set_dust_grid).  Returns 54 on a grid error. */
+typedef struct ddyn_grid_optics ddyn_grid_optics;
+int  ddyn_grid_optics_make(const ddyn_config *cfg, const ddyn_optics *o, ddyn_grid_optics **out);
+void ddyn_grid_optics_free(ddyn_grid_optics *t);
+/* Views of the tables for ddyn_upload_optics (valid while t lives). */
+int  ddyn_grid_optics_tables(const ddyn_grid_optics *t, ddyn_optics_tables *v);
+
+/* Wavelength indices of the .param file's image wavelengths (nwavs entries;
+ * src/functions.cu:329-335): with i the first optical wavelength at or above
+ * the requested one (at most the second last), the image is made at index
+ * i - 1, the wavelength below it (0 where the reference would use -1).
+ * Returns 43 for a wavelength outside the optical data. */
+int ddyn_image_wavelengths(const ddyn_config *cfg, const ddyn_grid_optics *t, int32_t *wj);
+
+/* Write an image as a FITS file <stub>_<wvl [um] %.2f>_<time %.3f>.fits
+ * (replaces Grid::write_image, src/fits.cu:9-85): a primary array of
+ * xsize x ysize FP64 pixels, F in ddyn_calc_flux's layout (pixel (x, y) at
+ * F[ysize * x + y]), with the keys WAVELENG [um], FLUX (the sum of all pixels
+ * [Jy]), PIXELSCA [arcsec/pixel], VERSION, a HISTORY line and DATE.  An
+ * existing file is replaced.  name (may be NULL, name_len bytes) gets the
+ * file name.  Returns 0, or 39 if the file cannot be written. */
+int ddyn_write_image(const char *version, const ddyn_config *cfg, double time, double wvl, const double *F,
+                     char *name, int32_t name_len);
+
+/* Write the SED <stub>_SED_<time %.2f>.dat (replaces Grid::write_sed,
+ * src/grid.cu:675-691): per wavelength "<w [um] %.3f> <stellar flux [Jy]
+ * %.6e> <grain flux [Jy] %.6e>".  in is the stellar flux at the observer per
+ * wavelength [W m^-3] (ddyn_optics_tables' in), Fexc ddyn_calc_sed's totals.
+ * Returns 0, or 39 if the file cannot be written. */
+int ddyn_write_sed(const char *version, const ddyn_config *cfg, double time, int32_t n_wav, const double *w,
+                   const double *in, const double *Fexc, char *name, int32_t name_len);
 
 /* Particle counts of the disks and blobs: gravitating bodies (sum of Ngrav) and
  * grain tracers (per disk or blob, bins x round(Nsmall / bins)); needs the dust

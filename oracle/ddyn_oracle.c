@@ -624,6 +624,117 @@ void orc_recalc_com(int64_t n_planets, double *x, double *y, double *z,
     vx[0] = Vx / mu[0]; vy[0] = Vy / mu[0]; vz[0] = Vz / mu[0];
 }
 
+/* ------------------------------------------------------------------------
+ * Images and SEDs: src/gpu_functions.cu:983-1156, one grain at a time in
+ * grain order, with the reference's expressions and its operation order.
+ * ---------------------------------------------------------------------- */
+#define ORC_AU 1.49598e11           /* src/ddyn.cuh:18 */
+#define ORC_CSPEED 299792458.0      /* :23 */
+#define ORC_KBOLTZ 1.3806503E-23    /* :24 */
+#define ORC_HPLANK 6.626068E-34     /* :25 */
+#define ORC_PI 3.14159265358979323846
+
+/* Grain temperature at R [AU] (src/gpu_functions.cu:1050-1064); the index is
+ * kept inside the table where rounding at R just below Rmax could step past it. */
+static double orc_temp(const orc_optics *o, int id, double R)
+{
+    const double f = log(o->Rmax / o->Rmin) / ((double)o->n_r - 1.0);
+    const double *Td = o->Td + (int64_t)id * o->n_r;
+    if (R <= o->Rmin) return Td[0];
+    if (R >= o->Rmax) return Td[o->n_r - 1];
+    int Rid = (int)(log(R / o->Rmin) / f) + 1;
+    if (Rid > o->n_r - 1) Rid = o->n_r - 1;
+    const double Rin = o->Rmin * exp((Rid - 1) * f);
+    const double Rout = o->Rmin * exp(Rid * f);
+    return exp(log(Td[Rid - 1]) + log(Td[Rid] / Td[Rid - 1]) * log(R / Rin) / log(Rout / Rin));
+}
+
+/* Flux [Jy] at wavelength j (src/gpu_functions.cu:1065-1080). */
+static double orc_flux(const orc_optics *o, double dist, int id, double n, double R, double cossc, double T, int j)
+{
+    const double wvl = o->w[j];
+    const double I = (2 * ORC_HPLANK * ORC_CSPEED * ORC_CSPEED) /
+                     (pow(wvl, 5) * (exp(ORC_HPLANK * ORC_CSPEED / (wvl * ORC_KBOLTZ * T)) - 1.0)) * ORC_PI;
+    int cosinm = (int)floor(0.5 * (cossc + 1.0) * (o->n_theta - 1));
+    if (cosinm < 0) cosinm = 0;
+    if (cosinm > o->n_theta - 2) cosinm = o->n_theta - 2;
+    const int cosinp = cosinm + 1;
+    const double cosm = (double)(cosinm)*2.0 / (o->n_theta - 1) - 1.0;
+    const double cosp = (double)(cosinp)*2.0 / (o->n_theta - 1) - 1.0;
+    const double Pm = o->Qpfunc[(int64_t)o->n_wav * ((int64_t)o->n_theta * id + cosinm) + j];
+    const double Pp = o->Qpfunc[(int64_t)o->n_wav * ((int64_t)o->n_theta * id + cosinp) + j];
+    const double P = exp(log(Pm) + (log(Pp) - log(Pm)) * (cossc - cosm) / (cosp - cosm));
+    const double s = o->size[id];
+    const double Fscat = o->in[j] * ORC_PI * s * s * o->Qsca[(int64_t)o->n_wav * id + j] * P / R / R / ORC_AU / ORC_AU;
+    const double Fther = I * s * s * o->Qabs[(int64_t)o->n_wav * id + j] / dist / dist;
+    return n * (Fther + Fscat) * 1e26 * wvl * wvl / ORC_CSPEED;
+}
+
+/* Scattering cosine (src/gpu_functions.cu:1010-1023). */
+static double orc_cossc(const orc_optics *o, const orc_view *v, double x, double y, double z, double R)
+{
+    const double ly = -v->dist * sin(v->incl) - y;
+    const double lz = v->dist * cos(v->incl) - z;
+    double c = (x * x + (-y) * ly + (-z) * lz) / R / sqrt(x * x + ly * ly + lz * lz);
+    if (c < -1) c = -1.0;
+    if (c >= 1.0 - 2.0 / o->n_theta) c = 1.0 - 2.0 / o->n_theta;
+    return c;
+}
+
+void orc_image(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n_dust, const double *x,
+               const double *y, const double *z, const double *N, const int32_t *id, int32_t wj, double *F)
+{
+    const int64_t npix = (int64_t)v->xsize * v->ysize;
+    for (int64_t k = 0; k < npix; k++) F[k] = 0.0;
+    for (int64_t i = 0; i < n_dust; i++) {
+        const int64_t t = n_grav + i;
+        const int b = id ? id[t] : -1;
+        if (b < 0 || b >= o->n_bins) continue;   /* (:994) */
+        double X = x[t], Y = y[t];
+        const double Z = z[t], n = N[t];
+        double phi = atan2(Y, X);
+        const double R = sqrt(X * X + Y * Y + Z * Z);
+        double Rd = sqrt(X * X + Y * Y);
+        X = Rd * cos(phi + v->rot);   /* (:1005-1006) */
+        Y = Rd * sin(phi + v->rot);
+        const double cossc = orc_cossc(o, v, X, Y, Z, R);
+        Y = sqrt(Y * Y + Z * Z) * cos(atan2(Z, Y) - v->incl);   /* (:1027) */
+        phi = atan2(Y, X);   /* (:1030-1034) */
+        Rd = sqrt(X * X + Y * Y);
+        X = Rd * cos(phi + v->PA);
+        Y = Rd * sin(phi + v->PA);
+        const double psca = v->xsize >= v->ysize ? v->fov / v->xsize : v->fov / v->ysize;   /* (:1036-1041) */
+        X = X + 0.5 * psca * v->xsize;
+        Y = Y + 0.5 * psca * v->ysize;
+        const double fx = floor(X / psca), fy = floor(Y / psca);
+        if (!(fx >= 0.0 && fx <= v->xsize - 1 && fy >= 0.0 && fy <= v->ysize - 1)) continue;   /* (:1047) */
+        if (!(R > o->Rmin)) continue;   /* (:1078) */
+        const double T = orc_temp(o, b, R);
+        F[(int64_t)v->ysize * (int64_t)fx + (int64_t)fy] += orc_flux(o, v->dist, b, n, R, cossc, T, wj);
+    }
+}
+
+void orc_sed(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n_dust, const double *x,
+             const double *y, const double *z, const double *N, const int32_t *id, double *Fexc)
+{
+    for (int j = 0; j < o->n_wav; j++) Fexc[j] = 0.0;
+    for (int64_t i = 0; i < n_dust; i++) {
+        const int64_t t = n_grav + i;
+        const int b = id ? id[t] : -1;
+        if (b < 0 || b >= o->n_bins) continue;   /* (:1092) */
+        double X = x[t], Y = y[t];
+        const double Z = z[t], n = N[t];
+        const double phi = atan2(Y, X);
+        const double R = sqrt(X * X + Y * Y + Z * Z);
+        X = R * cos(phi + v->rot);   /* the 3-D distance (:1104-1105) */
+        Y = R * sin(phi + v->rot);
+        const double cossc = orc_cossc(o, v, X, Y, Z, R);
+        if (!(R > o->Rmin)) continue;   /* Fdust = 0 (:1151-1154) */
+        const double T = orc_temp(o, b, R);
+        for (int j = 0; j < o->n_wav; j++) Fexc[j] += orc_flux(o, v->dist, b, n, R, cossc, T, j);
+    }
+}
+
 int orc_max_threads(void)
 {
 #ifdef _OPENMP

@@ -59,6 +59,25 @@ void orc_coe2rv(double M, double m, double a, double e, double inc, double w, do
                 double nu, double beta, double *out6);
 void orc_recalc_com(int64_t n_planets, double *x, double *y, double *z,
                     double *vx, double *vy, double *vz, const double *mu);
+/* Images and SEDs (src/gpu_functions.cu:983-1156): the dust grid's optical
+ * tables, in the layouts of the reference's device arrays. */
+typedef struct {
+    int32_t n_bins, n_wav, n_theta, n_r;
+    const double *w, *in, *size, *Qabs, *Qsca, *Qpfunc, *Td;   /* [wav], [wav], [bin], [bin][wav] x2,
+                                                                 [bin][theta][wav], [bin][radius] */
+    double Rmin, Rmax;                                          /* [AU] */
+} orc_optics;
+typedef struct {
+    double dist, rot, incl, PA, fov;   /* [m], [rad] x3, [AU] */
+    int32_t xsize, ysize;
+} orc_view;
+/* Image at wavelength index wj: F (xsize*ysize, pixel (x, y) at F[ysize*x + y])
+ * is zeroed, then every grain's flux is added to its pixel in grain order. */
+void orc_image(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n_dust, const double *x,
+               const double *y, const double *z, const double *N, const int32_t *id, int32_t wj, double *F);
+/* SED: Fexc (n_wav) = the sum over grains, in grain order, of every grain's flux. */
+void orc_sed(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n_dust, const double *x,
+             const double *y, const double *z, const double *N, const int32_t *id, double *Fexc);
 int  orc_max_threads(void);
 void orc_set_threads(int n);
 

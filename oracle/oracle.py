@@ -33,6 +33,17 @@ class OrcState(ctypes.Structure):
                 ("beta_p", _D), ("betasw_p", _D)]
 
 
+class OrcOptics(ctypes.Structure):
+    _fields_ = [("n_bins", ctypes.c_int32), ("n_wav", ctypes.c_int32), ("n_theta", ctypes.c_int32),
+                ("n_r", ctypes.c_int32)] + [(n, _D) for n in ("w", "in_", "size", "Qabs", "Qsca", "Qpfunc", "Td")] + \
+               [("Rmin", ctypes.c_double), ("Rmax", ctypes.c_double)]
+
+
+class OrcView(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("dist", "rot", "incl", "PA", "fov")] + \
+               [("xsize", ctypes.c_int32), ("ysize", ctypes.c_int32)]
+
+
 _lib = None
 
 
@@ -67,6 +78,12 @@ def lib():
         L.orc_coe2rv.restype = None
         L.orc_recalc_com.argtypes = [ctypes.c_int64] + [_D] * 7
         L.orc_recalc_com.restype = None
+        L.orc_image.argtypes = [ctypes.POINTER(OrcOptics), ctypes.POINTER(OrcView), ctypes.c_int64, ctypes.c_int64] + \
+            [_D] * 4 + [_I32, ctypes.c_int32, _D]
+        L.orc_image.restype = None
+        L.orc_sed.argtypes = [ctypes.POINTER(OrcOptics), ctypes.POINTER(OrcView), ctypes.c_int64, ctypes.c_int64] + \
+            [_D] * 4 + [_I32, _D]
+        L.orc_sed.restype = None
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -189,3 +206,32 @@ def max_threads():
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def _optics_view(t, v):
+    """C structs of a diskdyn_amd.imaging OpticsTables and View (same fields)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (t.w, t.in_, t.size, t.Qabs, t.Qsca, t.Qpfunc, t.Td)]
+    o = OrcOptics(t.Qabs.shape[0], t.Qabs.shape[1], t.Qpfunc.shape[1], t.Td.shape[1], *[_p(a) for a in arrs],
+                  float(t.Rmin), float(t.Rmax))
+    cv = OrcView(float(v.dist), float(v.rot), float(v.incl), float(v.PA), float(v.fov), int(v.xsize), int(v.ysize))
+    return o, cv, arrs
+
+
+def image(t, v, st, wj):
+    """Image at wavelength index wj, pixel (x, y) at F[ysize * x + y] [Jy]."""
+    o, cv, keep = _optics_view(t, v)
+    F = np.zeros(int(v.xsize) * int(v.ysize))
+    ids = None if st.id is None else np.ascontiguousarray(st.id, dtype=np.int32)
+    lib().orc_image(ctypes.byref(o), ctypes.byref(cv), st.n_grav, st.n_dust, _p(st.x), _p(st.y), _p(st.z), _p(st.N),
+                    _p(ids, _I32), int(wj), _p(F))
+    return F
+
+
+def sed(t, v, st):
+    """Grain flux [Jy] at every wavelength of the tables."""
+    o, cv, keep = _optics_view(t, v)
+    out = np.zeros(t.Qabs.shape[1])
+    ids = None if st.id is None else np.ascontiguousarray(st.id, dtype=np.int32)
+    lib().orc_sed(ctypes.byref(o), ctypes.byref(cv), st.n_grav, st.n_dust, _p(st.x), _p(st.y), _p(st.z), _p(st.N),
+                  _p(ids, _I32), _p(out))
+    return out

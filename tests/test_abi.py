@@ -46,6 +46,14 @@ def test_params_struct_layout():
     assert engine.DDynParams.fp64_radius.offset == 16 + 7 * 8
 
 
+def test_imaging_struct_layouts():
+    # ddyn_optics_tables: int32 x4, seven pointers, two doubles; ddyn_view: five doubles, int32 x2
+    from diskdyn_amd.imaging import COpticsTables, CView
+    assert ctypes.sizeof(COpticsTables) == 16 + 7 * 8 + 16
+    assert COpticsTables.w.offset == 16 and COpticsTables.Rmin.offset == 16 + 7 * 8
+    assert ctypes.sizeof(CView) == 5 * 8 + 8 and CView.xsize.offset == 40
+
+
 def test_engine_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(engine, "_lib", None)
     with pytest.raises(engine.DDynError):
