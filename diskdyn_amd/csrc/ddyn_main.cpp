@@ -8,16 +8,19 @@
 // Grid::evolve (src/ddyn.cu:37-111, src/grid.cu:62-310, 344-439): the optical
 // data and the grain tables of the dust grid (with grains), the particles of
 // the disks and blobs, the same time step (det_step),
-// the same clipping of steps to the data/FITS/SED output times, and data files
-// <stub>_<time>.dat written at the data cadence and at the end.  Imaging and
-// SEDs are not produced; their output times still clip the step so the
-// sequence of steps matches the reference.
+// the same clipping of steps to the data/FITS/SED output times, data files
+// <stub>_<time>.dat written at the data cadence and at the end, and with the
+// optical data the images <stub>_<wavelength>_<time>.fits at the FITS cadence
+// and the SEDs <stub>_SED_<time>.dat at the SED cadence (src/grid.cu:344-439:
+// images, SED and data file of the initial state of a .param run; then at each
+// output time the SED, the images and the data file, in that order).
 //
 // Options: --fp32 (FP32-force form), --fp64-rsq (FP64 with refined hardware
 //          reciprocal square roots in every pair law), --eps <AU> (softening of
 //          swarm pairs), --r64 <AU> (FP32-force form: swarm pairs closer than
 //          this are evaluated in FP64; default 10 eps, or 0.01 AU without
-//          softening), --rmin <AU> (sublimation radius), --device <n>, --quiet.
+//          softening), --rmin <AU> (sublimation radius), --no-images (data
+//          files only), --device <n>, --quiet.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -39,7 +42,7 @@ void usage()
 {
     std::fprintf(stderr,
                  "Usage: ddyn -i <init.param> | -m <model.dat> [--fp32 | --fp64-rsq] [--eps AU] [--r64 AU] [--rmin AU] "
-                 "[--device n] [--quiet]\n");
+                 "[--no-images] [--device n] [--quiet]\n");
 }
 
 struct State {
@@ -65,7 +68,7 @@ int main(int argc, char** argv)
 {
     const char* param = nullptr;
     const char* model = nullptr;
-    int precision = DDYN_FP64, device = 0, quiet = 0;
+    int precision = DDYN_FP64, device = 0, quiet = 0, images = 1;
     double eps = 0.0, rmin = 0.0, r64 = 0.0;
     for (int a = 1; a < argc; ++a) {
         const std::string s = argv[a];
@@ -78,6 +81,7 @@ int main(int argc, char** argv)
         else if (s == "--r64" && a + 1 < argc) r64 = std::atof(argv[++a]);
         else if (s == "--device" && a + 1 < argc) device = std::atoi(argv[++a]);
         else if (s == "--quiet") quiet = 1;
+        else if (s == "--no-images") images = 0;
         else { usage(); return s == "-h" ? 0 : 1; }
     }
     if (!param == !model) { usage(); return 1; }
@@ -93,6 +97,23 @@ int main(int argc, char** argv)
     double rmin_optics = 0.0;
     bool have_tables = false;
     ddyn_optics* optics = nullptr;
+    // the dust grid's optical tables for images and SEDs, and the image wavelengths
+    ddyn_grid_optics* gopt = nullptr;
+    std::vector<int32_t> wj;
+    auto load_imaging = [&]() {
+        if (!images || !optics) return;
+        int r = ddyn_grid_optics_make(&cfg, optics, &gopt);
+        if (!r) {
+            wj.assign((size_t)std::max(cfg.nwavs, 1), 0);
+            r = ddyn_image_wavelengths(&cfg, gopt, wj.data());
+            wj.resize((size_t)std::max(cfg.nwavs, 0));
+        }
+        if (r) {
+            std::fprintf(stderr, "ddyn: images and SEDs: %s; not written\n", ddyn_host_error_text(r));
+            ddyn_grid_optics_free(gopt);
+            gopt = nullptr;
+        }
+    };
     auto load_tables = [&]() -> int {
         const size_t n = (size_t)std::max(cfg.n_grid, 0);
         tb.assign(n, 0.f); tbsw.assign(n, 0.f); tq.assign(n, 0.f);
@@ -126,6 +147,7 @@ int main(int argc, char** argv)
                 ddyn_optics_free(optics);
                 return 1;
             }
+            load_imaging();
         }
         int64_t n_mass = 0, n_dust = 0;
         ddyn_generate_counts(&cfg, &n_mass, &n_dust);
@@ -165,6 +187,7 @@ int main(int argc, char** argv)
             // from the optical data on the file's dust grid, the bins from the masses
             if (ddyn_optics_read(cfg.comp, cfg.specfile, &cfg, &optics) == 0 && load_tables() == 0) {
                 ddyn_restart_ids(&cfg, run.n_all, st.mu.data(), st.id.data());
+                load_imaging();
             } else if (cfg.SWQ == 1 || cfg.FLQ == 1) {
                 // the data file carries beta per grain only: without the optical
                 // data the wind drag and the Lorentz force would silently be zero
@@ -229,6 +252,54 @@ int main(int argc, char** argv)
             return 1;
         }
     }
+    ddyn_optics_tables otab{};
+    if (gopt) {
+        ddyn_grid_optics_tables(gopt, &otab);
+        if (check(ctx, ddyn_upload_optics(ctx, &otab), "upload_optics")) {
+            ddyn_destroy(ctx);
+            ddyn_grid_optics_free(gopt);
+            return 1;
+        }
+    }
+    ddyn_view view{};
+    view.dist = cfg.dist;
+    view.rot = cfg.longitude;
+    view.incl = cfg.inc;
+    view.PA = cfg.PA;
+    view.fov = cfg.fov;
+    view.xsize = cfg.xsize;
+    view.ysize = cfg.ysize;
+    std::vector<double> img, sed;
+    // Grid::calc_flux + write_image per image wavelength (src/grid.cu:43-54, src/fits.cu)
+    auto write_images = [&](double time) -> int {
+        if (!gopt) return 0;
+        img.resize((size_t)std::max<int64_t>((int64_t)view.xsize * view.ysize, 1));
+        for (const int32_t j : wj) {
+            if (check(ctx, ddyn_calc_flux(ctx, &view, j, img.data()), "calc_flux")) return 1;
+            char name[600];
+            const int r = ddyn_write_image(kVersion, &cfg, time, otab.w[j], img.data(), name, (int32_t)sizeof name);
+            if (r) { std::fprintf(stderr, "ddyn: %s: %s\n", name, ddyn_host_error_text(r)); return 1; }
+            if (!quiet) std::printf("ddyn: wrote %s\n", name);
+        }
+        return 0;
+    };
+    // Grid::calc_flux_sed + copy_sed per wavelength, write_sed (src/grid.cu:56-65, 675-691)
+    auto write_sed = [&](double time) -> int {
+        if (!gopt) return 0;
+        sed.resize((size_t)otab.n_wav);
+        if (check(ctx, ddyn_calc_sed(ctx, &view, sed.data()), "calc_sed")) return 1;
+        char name[600];
+        const int r = ddyn_write_sed(kVersion, &cfg, time, otab.n_wav, otab.w, otab.in, sed.data(), name,
+                                     (int32_t)sizeof name);
+        if (r) { std::fprintf(stderr, "ddyn: %s: %s\n", name, ddyn_host_error_text(r)); return 1; }
+        if (!quiet) std::printf("ddyn: wrote %s\n", name);
+        return 0;
+    };
+    auto finish = [&](int code) {
+        ddyn_destroy(ctx);
+        ddyn_grid_optics_free(gopt);
+        return code;
+    };
     auto write = [&](double time) -> int {
         if (check(ctx, ddyn_download_state(ctx, st.x.data(), st.y.data(), st.z.data(), st.vx.data(), st.vy.data(),
                                            st.vz.data(), st.N.data()), "download_state"))
@@ -249,7 +320,7 @@ int main(int argc, char** argv)
     run.dt = dt;
     double dttmp = dt;
     if (!quiet) std::printf("ddyn: dt = %.8e yr, t_end = %.3f yr\n", dt, cfg.t_end);
-    if (param_mode && write(run.time)) { ddyn_destroy(ctx); return 1; }
+    if (param_mode && (write_images(run.time) || write_sed(run.time) || write(run.time))) return finish(1);
     auto clip = [&]() {
         if (run.time + dttmp > run.tlastd + cfg.t_write) dttmp = cfg.t_write + run.tlastd - run.time;
         if (run.time + dttmp > run.tlastf + cfg.t_fits) dttmp = cfg.t_fits + run.tlastf - run.time;
@@ -259,30 +330,39 @@ int main(int argc, char** argv)
     clip();
     int64_t steps = 0;
     // The time bookkeeping is host arithmetic only, so consecutive steps of the
-    // same size with no data file between them go to the GPU as one ddyn_step
+    // same size with no output between them go to the GPU as one ddyn_step
     // call (same steps, same results, and the library can overlap or fuse them).
     constexpr int64_t kMaxRun = 4096;
     do {
         const double h = dttmp;
         int64_t n = 0;
-        bool out = false;
+        bool out_sed = false, out_fits = false, out_data = false;
         do {
             ++n;
             run.time += h;
-            if (run.time >= run.tlasts + cfg.t_sed || run.time >= cfg.t_end) { run.tlasts = run.time; dttmp = dt; }
-            if (run.time >= run.tlastf + cfg.t_fits || run.time >= cfg.t_end) { run.tlastf = run.time; dttmp = dt; }
+            if (run.time >= run.tlasts + cfg.t_sed || run.time >= cfg.t_end) {
+                run.tlasts = run.time;
+                out_sed = true;
+                dttmp = dt;
+            }
+            if (run.time >= run.tlastf + cfg.t_fits || run.time >= cfg.t_end) {
+                run.tlastf = run.time;
+                out_fits = true;
+                dttmp = dt;
+            }
             if (run.time >= run.tlastd + cfg.t_write || run.time >= cfg.t_end) {
                 run.tlastd = run.time;
-                out = true;
+                out_data = true;
                 dttmp = dt;
             }
             clip();
-        } while (!out && dttmp == h && run.time < cfg.t_end && n < kMaxRun);
-        if (check(ctx, ddyn_step(ctx, h, n), "step")) { ddyn_destroy(ctx); return 1; }
+        } while (!out_sed && !out_fits && !out_data && dttmp == h && run.time < cfg.t_end && n < kMaxRun);
+        if (check(ctx, ddyn_step(ctx, h, n), "step")) return finish(1);
         steps += n;
-        if (out && write(run.time)) { ddyn_destroy(ctx); return 1; }
+        if (out_sed && write_sed(run.time)) return finish(1);
+        if (out_fits && write_images(run.time)) return finish(1);
+        if (out_data && write(run.time)) return finish(1);
     } while (run.time < cfg.t_end);
     if (!quiet) std::printf("ddyn: %lld steps to t = %.6f yr\n", (long long)steps, run.time);
-    ddyn_destroy(ctx);
-    return 0;
+    return finish(0);
 }

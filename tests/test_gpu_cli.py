@@ -176,3 +176,58 @@ def test_param_run_generates_disks_and_blobs(tmp_path):
     assert r.returncode == 0
     _, _, fin2 = hostio.read_datafile(tmp_path / "restart" / "setupstub_1.00.dat")
     assert np.abs(fin2.pos() - fin.pos()).max() < 1e-6 * np.abs(fin.pos()).max()
+
+
+def test_param_run_writes_images_and_seds(tmp_path):
+    """The same kind of run writes, at t = 0 and at every FITS and SED output
+    time, one FITS image per image wavelength and one SED file.  The initial
+    image is the oracle's image of the generated state (with the host's grid
+    optics and the .param observer) pixel for pixel, the initial SED its SED
+    to the file's %.6e; the final images carry the total flux of the oracle's
+    image of the final data file's state; --no-images writes data files only."""
+    from diskdyn_amd.imaging import image_to_fits_rows, view_from_config
+    from tests import opticslib as OL
+    from tests.test_imaging_host import _read_fits
+    comp, spec = _write_optics(tmp_path)
+    disks = "30.0 40.0 0.05 0.05 100.0 1.0 3.67 0.5 -1.0 50 3000"
+    blobs = "35.0 0.1 5.0 10.0 20.0 30.0 100.0 1.0 3.67 0.1 0 1000 collision 3.0"
+    p = OL.setup_param(tmp_path / "img.param", comp, spec, disks, blobs, physics="yes yes no", t_end=1.0)
+    p.write_text(p.read_text().replace("1.0 1.0 1.0\n", "0.5 0.5 0.5\n", 1))
+    r = subprocess.run([DDYN, "-i", str(p)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    cfg = hostio.read_param(p)
+    opt = hostio.Optics(cfg.comp.decode(), cfg.specfile.decode(), cfg)
+    hostio.set_dust_grid(cfg, opt)
+    tab = hostio.grain_tables(cfg, opt)
+    tables, wj = hostio.grid_optics(cfg, opt)
+    view = view_from_config(cfg)
+    assert len(wj) == 2
+    st0 = hostio.generate(cfg, opt, tab["beta"])
+    for t in (0.0, 0.5, 1.0):
+        assert os.path.exists(tmp_path / f"setupstub_SED_{t:.2f}.dat")
+        for j in wj:
+            assert os.path.exists(tmp_path / f"setupstub_{tables.w[j] * 1e6:.2f}_{t:.3f}.fits")
+    for j in wj:
+        cards, data = _read_fits(tmp_path / f"setupstub_{tables.w[j] * 1e6:.2f}_0.000.fits")
+        ref = image_to_fits_rows(O.image(tables, view, st0, j), view.xsize, view.ysize)
+        assert np.array_equal(data > 0, ref > 0) and (ref > 0).sum() > 100
+        nz = ref > 0
+        assert np.abs(data[nz] / ref[nz] - 1.0).max() <= 1e-12
+        assert float(cards["FLUX"]) == pytest.approx(ref.sum(), rel=1e-12)
+    rows = np.loadtxt(tmp_path / "setupstub_SED_0.00.dat")
+    sed0 = O.sed(tables, view, st0)
+    assert rows.shape == (tables.n_wav, 3)
+    assert np.allclose(rows[:, 2], sed0, rtol=6e-7, atol=0)
+    _, _, fin = hostio.read_datafile(tmp_path / "setupstub_1.00.dat")
+    fin.id = st0.id
+    for j in wj:
+        cards, data = _read_fits(tmp_path / f"setupstub_{tables.w[j] * 1e6:.2f}_1.000.fits")
+        total = O.image(tables, view, fin, j).sum()
+        assert float(cards["FLUX"]) == pytest.approx(total, rel=1e-6)
+    (tmp_path / "noimg").mkdir()
+    r = subprocess.run([DDYN, "-i", str(p), "--no-images", "--quiet"], cwd=tmp_path / "noimg", capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0
+    names = sorted(os.listdir(tmp_path / "noimg"))
+    assert names == ["setupstub_0.00.dat", "setupstub_0.50.dat", "setupstub_1.00.dat"]
