@@ -19,7 +19,8 @@ events on the stream it runs on, against the FP32 vector peak.  The CPU
 baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample
 of whole RK4 steps.  The self-gravitating swarm (65,537 bodies), the large
 disk (1,048,577 bodies) and the debris ring with 1,048,576 grains run too, at
-reduced step counts, under "also".
+reduced step counts, under "also", with images and SEDs of the debris ring's
+grains ("also.imaging").
 
 Prints one JSON line.
 """
@@ -257,6 +258,53 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count()}
 
 
+def imaging_run(n_dust=262144, xsize=512, ysize=512, calls=20):
+    """Images and SEDs of the debris ring's grains (synthetic optics, 48
+    wavelengths, an inclined observer at 10 pc): milliseconds per
+    ddyn_calc_flux call (one 512 x 512 image, including its 2 MB copy to the
+    host) and per ddyn_calc_sed call (all wavelengths), host clock over
+    `calls` calls after two untimed ones; each call synchronises.  The oracle
+    image and SED of the same state, single-threaded on this host, are timed
+    beside them and give the largest relative difference."""
+    from oracle import oracle as O
+    w, _ = workload("cfg3", FP32_FORCE, n_dust=n_dust)
+    t, v = synth.ring_imaging(w, xsize=xsize, ysize=ysize)
+    st = w.state
+    st.N[st.n_grav:] = 10.0 ** np.random.default_rng(3).uniform(10.0, 25.0, st.n_dust)
+    wj = t.n_wav // 2
+    with Engine(0) as eng:
+        eng.upload(st.copy(), w.params)
+        eng.upload_optics(t)
+        for _ in range(2):
+            F = eng.calc_flux(v, wj)
+            S = eng.calc_sed(v)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            F = eng.calc_flux(v, wj)
+        t_img = (time.perf_counter() - t0) / calls
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            S = eng.calc_sed(v)
+        t_sed = (time.perf_counter() - t0) / calls
+    t0 = time.perf_counter()
+    Fr = O.image(t, v, st, wj)
+    c_img = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Sr = O.sed(t, v, st)
+    c_sed = time.perf_counter() - t0
+    nz = Fr > 0
+    return {
+        "config": {"workload": "cfg3 debris ring grains, imaging", "n_dust": n_dust, "image": [xsize, ysize],
+                   "n_wav": t.n_wav, "n_theta": t.n_theta, "n_bins": t.n_bins},
+        "image_ms": t_img * 1e3, "image_grains_per_s": n_dust / t_img,
+        "sed_ms": t_sed * 1e3, "sed_grain_wavelengths_per_s": n_dust * t.n_wav / t_sed,
+        "max_rel_err_vs_oracle": {"image": float(np.max(np.abs(F[nz] / Fr[nz] - 1.0))),
+                                  "sed": float(np.max(np.abs(S / Sr - 1.0)))},
+        "cpu_baseline": {"image_ms": c_img * 1e3, "sed_ms": c_sed * 1e3, "cores": 1, "kind": "port",
+                         "sample": "the oracle's image and SED of the same state (whole)"},
+    }
+
+
 def adjoint_run(name, precision=FP32_FORCE):
     """Discrete-adjoint workloads.
 
@@ -454,7 +502,10 @@ def main():
                     help="steps for the FP64 comparison runs (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-also", action="store_true",
-                    help="skip the swarm, large-disk and 1,048,576-grain lines that the debris-ring run adds under 'also'")
+                    help="skip the swarm, large-disk, 1,048,576-grain and imaging lines that the debris-ring run "
+                         "adds under 'also'")
+    ap.add_argument("--imaging", action="store_true",
+                    help="time only the images and SEDs of the debris ring's grains (the 'also.imaging' entry)")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the discrete adjoint instead (workloads cfg4 and cfg5)")
     ap.add_argument("--fp64", action="store_true", help="with --adjoint cfg5: the plain FP64 form")
@@ -472,6 +523,9 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the swarm would shard by target block, the grains by grain range)", file=sys.stderr)
         return 2
+    if args.imaging:
+        print(json.dumps({"imaging": imaging_run()}), flush=True)
+        return 0
     if args.adjoint:
         if args.workload not in ("cfg4", "cfg5"):
             raise SystemExit("--adjoint runs the cfg4 or cfg5 workload")
@@ -511,6 +565,7 @@ def main():
             also[key]["fp64"] = {k: a["fp64"][k] for k in ("value", "ms_per_step", "steps", "kernel_frac_fp64_peak")}
             also[key]["fp64_rsq"] = {k: a["fp64_rsq"][k] for k in ("value", "ms_per_step", "steps",
                                                                     "speedup_fp32_vs_fp64_rsq")}
+        also["imaging"] = imaging_run()
         out["also"] = also
     print(json.dumps(out), flush=True)
     return 0

@@ -39,6 +39,9 @@ struct ImageData {
     double Rmin = 0.0, Rmax = 0.0;
     double *w = nullptr, *in = nullptr, *size = nullptr, *Qabs = nullptr, *Qsca = nullptr, *Qpf = nullptr,
            *Td = nullptr;
+    // derived on the device at upload (k_optics_prep): per wavelength lambda^5 and
+    // lambda k_B, and the logarithm of every phase-function entry
+    double *w5 = nullptr, *wk = nullptr, *lQp = nullptr;
     // scratch, grown as needed
     size_t cap_dust = 0;
     uint32_t *key = nullptr, *key_s = nullptr;
@@ -68,7 +71,8 @@ struct ImgArgs {
     int64_t n_grav, n_dust;
     const double *x, *y, *z, *N;
     const int32_t* id;
-    const double *w, *in, *size, *Qabs, *Qsca, *Qpf, *Td;
+    const double *w, *in, *size, *Qabs, *Qsca, *lQp, *Td;
+    const double *w5, *wk;
     int32_t n_bins, n_wav, n_theta, n_r;
     double Rmin, Rmax;
     double dist, rot, incl, PA, fov;
@@ -92,18 +96,21 @@ __device__ double grain_temp(const ImgArgs& A, int id, double R)
 
 // Flux [Jy] of one grain (tracer count n, bin id, distance R [AU], scattering
 // cosine cs, temperature T) at wavelength j (src/gpu_functions.cu:1065-1080).
+// The wavelength's lambda^5 and lambda k_B and the logarithms of the phase
+// function come from tables made at upload with the same operations, so the
+// results are those of forming them here.
 __device__ double grain_flux(const ImgArgs& A, int id, double n, double R, double cs, double T, int j)
 {
     const double wvl = A.w[j];
     const double I = (2 * kPlanck * kCspeed * kCspeed) /
-                     (pow(wvl, 5) * (exp(kPlanck * kCspeed / (wvl * kBoltz * T)) - 1.0)) * kPI;   // pi B_lambda
+                     (A.w5[j] * (exp(kPlanck * kCspeed / (A.wk[j] * T)) - 1.0)) * kPI;   // pi B_lambda
     // (the clamp only matters for a non-finite cosine, from a non-finite position)
     const int cosinm = (int)fmin(fmax(floor(0.5 * (cs + 1.0) * (A.n_theta - 1)), 0.0), (double)(A.n_theta - 2));
     const int cosinp = cosinm + 1;
     const double cosm = (double)(cosinm)*2.0 / (A.n_theta - 1) - 1.0;
     const double cosp = (double)(cosinp)*2.0 / (A.n_theta - 1) - 1.0;
-    const double* Qp = A.Qpf + (int64_t)A.n_wav * ((int64_t)A.n_theta * id + cosinm) + j;
-    const double lm = log(Qp[0]), lp = log(Qp[A.n_wav]);
+    const double* lQp = A.lQp + (int64_t)A.n_wav * ((int64_t)A.n_theta * id + cosinm) + j;
+    const double lm = lQp[0], lp = lQp[A.n_wav];
     const double P = exp(lm + (lp - lm) * (cs - cosm) / (cosp - cosm));
     const double s = A.size[id];
     const int64_t q = (int64_t)A.n_wav * id + j;
@@ -242,6 +249,20 @@ __global__ __launch_bounds__(kImgWG) void k_sed_reduce(const double* part, int64
     if (t == 0) sed[j] = sh[0];
 }
 
+// Tables derived from the uploaded ones: pow(lambda, 5) and lambda k_B per
+// wavelength (the first factors of the Planck function as the reference forms
+// it, (lambda k_B) T), and log of every phase-function entry.
+__global__ void k_optics_prep(const double* w, int n_wav, const double* Qpf, int64_t n_pf, double* w5, double* wk,
+                              double* lQp)
+{
+    const int64_t i = (int64_t)blockIdx.x * kImgWG + threadIdx.x;
+    if (i < n_wav) {
+        w5[i] = pow(w[i], 5);
+        wk[i] = w[i] * kBoltz;
+    }
+    if (i < n_pf) lQp[i] = log(Qpf[i]);
+}
+
 int fail(ddyn_ctx* c, int code, const std::string& msg)
 {
     c->err = msg;
@@ -269,6 +290,7 @@ void dfree(T*& p)
 void free_tables(ImageData* d)
 {
     dfree(d->w); dfree(d->in); dfree(d->size); dfree(d->Qabs); dfree(d->Qsca); dfree(d->Qpf); dfree(d->Td);
+    dfree(d->w5); dfree(d->wk); dfree(d->lQp);
 }
 
 // Grows a scratch array to at least n elements (contents not kept).
@@ -299,7 +321,8 @@ int prepare(ddyn_ctx* c, const ddyn_view* v, bool image, ImgArgs& A)
     A.n_grav = c->n_grav;
     A.n_dust = c->n_dust;
     A.x = c->x; A.y = c->y; A.z = c->z; A.N = c->N; A.id = c->id;
-    A.w = d->w; A.in = d->in; A.size = d->size; A.Qabs = d->Qabs; A.Qsca = d->Qsca; A.Qpf = d->Qpf; A.Td = d->Td;
+    A.w = d->w; A.in = d->in; A.size = d->size; A.Qabs = d->Qabs; A.Qsca = d->Qsca; A.lQp = d->lQp; A.Td = d->Td;
+    A.w5 = d->w5; A.wk = d->wk;
     A.n_bins = d->n_bins; A.n_wav = d->n_wav; A.n_theta = d->n_theta; A.n_r = d->n_r;
     A.Rmin = d->Rmin; A.Rmax = d->Rmax;
     A.dist = v->dist; A.rot = v->rot; A.incl = v->incl; A.PA = v->PA; A.fov = v->fov;
@@ -343,15 +366,22 @@ int ddyn_upload_optics(ddyn_ctx* c, const ddyn_optics_tables* t)
     const size_t nb = (size_t)t->n_bins, nw = (size_t)t->n_wav, nt = (size_t)t->n_theta, nr = (size_t)t->n_r;
     struct { double** dst; const double* src; size_t n; } cp[] = {
         {&d->w, t->w, nw}, {&d->in, t->in, nw}, {&d->size, t->size, nb}, {&d->Qabs, t->Qabs, nb * nw},
-        {&d->Qsca, t->Qsca, nb * nw}, {&d->Qpf, t->Qpfunc, nb * nt * nw}, {&d->Td, t->Td, nb * nr}};
+        {&d->Qsca, t->Qsca, nb * nw}, {&d->Qpf, t->Qpfunc, nb * nt * nw}, {&d->Td, t->Td, nb * nr},
+        {&d->w5, nullptr, nw}, {&d->wk, nullptr, nw}, {&d->lQp, nullptr, nb * nt * nw}};
     for (auto& e : cp) {
         if (hipMalloc((void**)e.dst, e.n * sizeof(double)) != hipSuccess) {
             *e.dst = nullptr;
             free_tables(d);
             return fail(c, DDYN_ERR_NOMEM, "optical tables: device allocation failed");
         }
-        IMG_HIP(c, hipMemcpy(*e.dst, e.src, e.n * sizeof(double), hipMemcpyHostToDevice));
+        if (e.src) IMG_HIP(c, hipMemcpy(*e.dst, e.src, e.n * sizeof(double), hipMemcpyHostToDevice));
     }
+    const int64_t npf = (int64_t)(nb * nt * nw);
+    const int64_t nprep = npf > (int64_t)nw ? npf : (int64_t)nw;
+    k_optics_prep<<<(unsigned)((nprep + kImgWG - 1) / kImgWG), kImgWG, 0, c->stream>>>(d->w, t->n_wav, d->Qpf, npf,
+                                                                                     d->w5, d->wk, d->lQp);
+    IMG_HIP(c, hipGetLastError());
+    IMG_HIP(c, hipStreamSynchronize(c->stream));
     d->n_bins = t->n_bins; d->n_wav = t->n_wav; d->n_theta = t->n_theta; d->n_r = t->n_r;
     d->Rmin = t->Rmin; d->Rmax = t->Rmax;
     return DDYN_OK;
