@@ -1,7 +1,8 @@
 """In-tree build of the native parts.
 
 * libddyn_hip.so  HIP kernels + C ABI (include/ddyn.h): ddyn_hip.hip (state,
-  steps, adjoint) and ddyn_image.hip (images and SEDs), hipcc for gfx950 with
+  steps, adjoint), ddyn_image.hip (images and SEDs) and ddyn_mie.hip (Mie
+  tables), hipcc for gfx950 with
   -O3, C++17, -ffp-contract=off (every fused multiply-add in the kernels is
   written out explicitly, so rounding is fixed) and the packed-FP32 target
   feature switched off.
@@ -9,6 +10,8 @@
   optical data, dust grid, grain tables and particle generation
   (include/ddyn_host.h), host C++ only.
 * ddyn            command-line driver linking both.
+* runmie          the Mie-table tool (refractive indices in, composition table
+                  out), linking libddyn_hip.so.
 """
 import os
 import subprocess
@@ -22,6 +25,7 @@ CXX = os.environ.get("CXX", "g++")
 LIB = os.path.join(HERE, "libddyn_hip.so")
 HOST_LIB = os.path.join(HERE, "libddyn_host.so")
 CLI = os.path.join(HERE, "ddyn")
+MIE_CLI = os.path.join(HERE, "runmie")
 HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + INC]
@@ -37,7 +41,7 @@ def _run(cmd, what):
 def build_hip(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "ddyn_hip.hip"), os.path.join(CSRC, "ddyn_image.hip"),
-                                "-o", tmp], "hipcc")
+                                os.path.join(CSRC, "ddyn_mie.hip"), "-o", tmp], "hipcc")
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
@@ -53,8 +57,12 @@ def build_host(verbose=False):
     _run([CXX] + HOST_FLAGS + [os.path.join(CSRC, "ddyn_main.cpp"), "-o", tmp, "-L" + HERE,
                                "-lddyn_hip", "-lddyn_host", "-Wl,-rpath,$ORIGIN"], "ddyn driver")
     os.replace(tmp, CLI)
+    tmp = MIE_CLI + ".tmp"
+    _run([CXX] + HOST_FLAGS + [os.path.join(CSRC, "ddyn_runmie.cpp"), "-o", tmp, "-L" + HERE, "-lddyn_hip",
+                               "-Wl,-rpath,$ORIGIN"], "runmie tool")
+    os.replace(tmp, MIE_CLI)
     if verbose:
-        print("built", HOST_LIB, "and", CLI)
+        print("built", HOST_LIB, ",", CLI, "and", MIE_CLI)
     return HOST_LIB
 
 

@@ -58,6 +58,8 @@ SIGNATURES = {
     "ddyn_upload_optics": (ctypes.c_int, [_VP, ctypes.POINTER(COpticsTables)]),
     "ddyn_calc_flux": (ctypes.c_int, [_VP, ctypes.POINTER(CView), ctypes.c_int32, _D]),
     "ddyn_calc_sed": (ctypes.c_int, [_VP, ctypes.POINTER(CView), _D]),
+    "ddyn_mie": (ctypes.c_int, [_VP, ctypes.c_int32, _D, ctypes.c_int32, _D, _D, _D, ctypes.c_int32, _D, _D, _D,
+                                _D]),
 }
 
 _lib = None
@@ -273,6 +275,16 @@ class Engine:
         self._check(self.lib.ddyn_calc_sed(self.ctx, ctypes.byref(v), _dp(out) if len(out) else None),
                     "ddyn_calc_sed")
         return out
+
+    def mie(self, s_um, lam_um, n, k, cost):
+        """Mie tables on the device: Qabs, Qsca (n_sizes, n_wav) and the phase
+        function (n_sizes, n_theta, n_wav) at the cosines cost."""
+        s, lam, nr, ki, c = (np.ascontiguousarray(a, dtype=np.float64) for a in (s_um, lam_um, n, k, cost))
+        ns, nw, nt = len(s), len(lam), len(c)
+        qa, qs, qp = np.empty(ns * nw), np.empty(ns * nw), np.empty(ns * nt * nw)
+        self._check(self.lib.ddyn_mie(self.ctx, ns, _dp(s), nw, _dp(lam), _dp(nr), _dp(ki), nt, _dp(c), _dp(qa),
+                                      _dp(qs), _dp(qp)), "ddyn_mie")
+        return qa.reshape(ns, nw), qs.reshape(ns, nw), qp.reshape(ns, nt, nw)
 
     def rsqrt_fp64(self, x):
         """The library's device FP64 reciprocal square root of every x > 0."""

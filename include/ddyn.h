@@ -272,6 +272,23 @@ int ddyn_calc_flux(ddyn_ctx *ctx, const ddyn_view *v, int32_t wj, double *F);
  * Synchronises. */
 int ddyn_calc_sed(ddyn_ctx *ctx, const ddyn_view *v, double *Fexc);
 
+/* Mie efficiencies and phase functions of homogeneous spheres: the tables
+ * the reference's tools/runmie computes with mie_single (tools/mie_single.c,
+ * tools/runmie.c), in FP64 on the device.  For every radius s_um[i] (n_sizes,
+ * microns) and wavelength lam_um[j] (n_wav, microns) with complex refractive
+ * index n_re[j] - i k_im[j]: Qabs, Qsca [i * n_wav + j] and the phase function
+ * Qpfunc [(i * n_theta + t) * n_wav + j] at the n_theta cosines cost[t] of the
+ * scattering angle, normalised to 1 over the sphere (the layout of a
+ * composition table, ddyn_optics_read).  n_theta <= 256, size parameters
+ * 2 pi s / lambda <= 7.2e6 (the reference's limit), n_sizes * n_wav <= 2^30,
+ * finite n_re > 0 and k_im >= 0.
+ * Uses the context's device and main stream only (no state needed);
+ * synchronises.  Negative efficiencies are returned as computed (the tool
+ * clamps them to 0 when writing). */
+int ddyn_mie(ddyn_ctx *ctx, int32_t n_sizes, const double *s_um, int32_t n_wav, const double *lam_um,
+             const double *n_re, const double *k_im, int32_t n_theta, const double *cost, double *Qabs,
+             double *Qsca, double *Qpfunc);
+
 /* y[i] = the library's FP64 reciprocal square root of x[i] > 0, evaluated on the
  * device (the v_rsq_f64 estimate refined by one third-order step, used by every
  * pair law of DDYN_FP64_RSQ and by the adjoint's Jacobian products).  n <= 2^30;

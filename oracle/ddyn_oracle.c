@@ -735,6 +735,125 @@ void orc_sed(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n_d
     }
 }
 
+/* ------------------------------------------------------------------------
+ * Mie scattering (tools/mie_single.c:54-224): the logarithmic derivative
+ * D_n(m x) by downward recurrence from n_max, the Riccati-Bessel functions
+ * psi_n, chi_n upward, the coefficients a_n, b_n, the efficiency sums and
+ * the amplitude sums S+ = S1 + S2, S- = S1 - S2 with the angular functions
+ * pi_n, tau_n, in the reference's order.  Complex numbers are written out as
+ * real and imaginary parts, as there.
+ * ---------------------------------------------------------------------- */
+#define ORC_MIE_PI 3.141592653589793238
+#define ORC_MIE_MAXX 7200000.0
+
+int orc_mie(double s, const double *lam, const double *nre, const double *kim, int nl, const double *cost, int nt,
+            double *Qabs, double *Qsca, double *Qpfunc)
+{
+    double *Spr = calloc((size_t)nt, sizeof(double)), *Spi = calloc((size_t)nt, sizeof(double));
+    double *Smr = calloc((size_t)nt, sizeof(double)), *Smi = calloc((size_t)nt, sizeof(double));
+    double *P0r = calloc((size_t)nt, sizeof(double)), *P0i = calloc((size_t)nt, sizeof(double));
+    double *P1r = calloc((size_t)nt, sizeof(double)), *P1i = calloc((size_t)nt, sizeof(double));
+    int rc = 0;
+    for (int i = 0; i < nl && rc == 0; i++) {   /* (:93) */
+        const double x = 2.0 * ORC_MIE_PI * s / lam[i];
+        if (x > ORC_MIE_MAXX) { rc = -1; break; }
+        const double cr = nre[i], ci = -kim[i];
+        const double Irr = cr / (cr * cr + ci * ci), Iri = -ci / (cr * cr + ci * ci);
+        const double Yr = x * cr, Yi = x * ci;
+        double nstop;   /* (:105-112) */
+        if (x < 0.02) nstop = 2.0;
+        else if (x <= 8.0) nstop = x + 4.0 * pow(x, 0.3333333333333333) + 2.0;
+        else if (x < 4200.0) nstop = x + 4.05 * pow(x, 0.3333333333333333) + 2.0;
+        else nstop = x + 4.0 * pow(x, 0.3333333333333333) + 2.0;
+        double t = sqrt(Yr * Yr + Yi * Yi);
+        t = (nstop > t) ? nstop : t;
+        t += 15.0;
+        const long nmax = (long)floor(t);
+        double *dr = calloc((size_t)nmax + 1, sizeof(double)), *di = calloc((size_t)nmax + 1, sizeof(double));
+        for (long n = nmax - 1; n > 0; n--) {   /* (:124-129) */
+            const double tr = (((double)(n + 1)) * Yr) / (Yr * Yr + Yi * Yi);
+            const double ti = (-((double)(n + 1)) * Yi) / (Yr * Yr + Yi * Yi);
+            const double ur = tr + dr[n + 1], ui = ti + di[n + 1];
+            dr[n] = tr - (ur / (ur * ur + ui * ui));
+            di[n] = ti - (-ui / (ur * ur + ui * ui));
+        }
+        for (int k = 0; k < nt; k++) { Spr[k] = Spi[k] = Smr[k] = Smi[k] = P0r[k] = P0i[k] = P1i[k] = 0.0; P1r[k] = 1.0; }
+        double Psi0 = cos(x), Psi1 = sin(x), Chi0 = -Psi1, Chi1 = Psi0;
+        double Dg = 0.0, Dqsc = 0.0, Dqxt = 0.0, Tnp1 = 1.0;
+        double Ar1 = 0.0, Ai1 = 0.0, Br1 = 0.0, Bi1 = 0.0;   /* a_{n-1}, b_{n-1} */
+        for (long n = 1; n <= nstop; n++) {   /* (:147) */
+            Tnp1 += 2.0;
+            const double Tnm1 = Tnp1 - 2.0;
+            const double DN = (double)n;
+            const double A2 = Tnp1 / (DN * (DN + 1.0));
+            const double Turbo = (DN + 1.0) / DN;
+            const double Rnx = n / x;
+            const double Psi = Tnm1 * (Psi1 / x) - Psi0;
+            const double Chi = Tnm1 * (Chi1 / x) - Chi0;
+            double nr, ni, qr, qi, q2r, q2i, den, Ar, Ai, Br, Bi;
+            /* a_n with D_n / m (:162-173) */
+            nr = ((dr[n] * Irr - di[n] * Iri) + Rnx) * Psi - Psi1;
+            ni = (dr[n] * Iri + di[n] * Irr) * Psi;
+            qr = ((dr[n] * Irr - di[n] * Iri) + Rnx);
+            qi = (dr[n] * Iri + di[n] * Irr);
+            q2r = qr * Psi - qi * Chi;
+            q2i = qr * Chi + qi * Psi;
+            qr = q2r - Psi1;
+            qi = q2i - Chi1;
+            den = 1. / (qr * qr + qi * qi);
+            Ar = (nr * qr + ni * qi) * den;
+            Ai = (ni * qr - nr * qi) * den;
+            /* b_n with m D_n (:176-187) */
+            nr = ((dr[n] * cr - di[n] * ci) + Rnx) * Psi - Psi1;
+            ni = (dr[n] * ci + di[n] * cr) * Psi;
+            qr = ((dr[n] * cr - di[n] * ci) + Rnx);
+            qi = (dr[n] * ci + di[n] * cr);
+            q2r = qr * Psi - qi * Chi;
+            q2i = qr * Chi + qi * Psi;
+            qr = q2r - Psi1;
+            qi = q2i - Chi1;
+            den = 1. / (qr * qr + qi * qi);
+            Br = (nr * qr + ni * qi) * den;
+            Bi = (ni * qr - nr * qi) * den;
+            Dqxt = Tnp1 * (Ar + Br) + Dqxt;
+            Dqsc = Tnp1 * (Ar * Ar + Ai * Ai + Br * Br + Bi * Bi) + Dqsc;
+            if (n > 1)
+                Dg = Dg + ((DN * DN - 1.0) / DN) * ((Ar1 * Ar + Ai1 * Ai) + (Br1 * Br + Bi1 * Bi)) +
+                     Tnm1 * (Ar1 * Br1 + Ai1 * Bi1) / (DN * DN - DN);
+            Ar1 = Ar; Ai1 = Ai; Br1 = Br; Bi1 = Bi;
+            for (int k = 0; k < nt; k++) {   /* (:201-219) */
+                const double Sr = cost[k] * P1r[k], Si = cost[k] * P1i[k];
+                const double Tr = Sr - P0r[k], Ti = Si - P0i[k];
+                const double Tar = n * Tr - P0r[k], Tai = n * Ti - P0i[k];
+                const double aAr = A2 * Ar, aAi = A2 * Ai, aBr = A2 * Br, aBi = A2 * Bi;
+                Spr[k] = (aAr + aBr) * (P1r[k] + Tar) - (aAi + aBi) * (P1i[k] + Tai) + Spr[k];
+                Spi[k] = (aAi + aBi) * (P1r[k] + Tar) + (aAr + aBr) * (P1i[k] + Tai) + Spi[k];
+                Smr[k] = (aAr - aBr) * (P1r[k] - Tar) - (aAi - aBi) * (P1i[k] - Tai) + Smr[k];
+                Smi[k] = (aAi - aBi) * (P1r[k] - Tar) + (aAr - aBr) * (P1i[k] - Tai) + Smi[k];
+                P0r[k] = P1r[k];
+                P0i[k] = P1i[k];
+                P1r[k] = Sr + Turbo * Tr;
+                P1i[k] = Si + Turbo * Ti;
+            }
+            Psi0 = Psi1; Psi1 = Psi; Chi0 = Chi1; Chi1 = Chi;
+        }
+        free(dr);
+        free(di);
+        for (int k = 0; k < nt; k++) {   /* (:232-241) */
+            const double X1r = (Spr[k] + Smr[k]) * 0.5, X1i = (Spi[k] + Smi[k]) * 0.5;
+            const double X2r = (Spr[k] - Smr[k]) * 0.5, X2i = (Spi[k] - Smi[k]) * 0.5;
+            Qpfunc[i + (long)nl * k] = ((X1r * X1r) + (X1i * X1i) + (X2r * X2r) + (X2i * X2i)) / (4 * ORC_MIE_PI * Dqsc);
+        }
+        Dqsc = 2.0 * Dqsc / (x * x);
+        Dqxt = 2.0 * Dqxt / (x * x);
+        Qabs[i] = Dqxt - Dqsc;
+        Qsca[i] = Dqsc;
+        (void)Dg;
+    }
+    free(Spr); free(Spi); free(Smr); free(Smi); free(P0r); free(P0i); free(P1r); free(P1i);
+    return rc;
+}
+
 int orc_max_threads(void)
 {
 #ifdef _OPENMP

@@ -78,6 +78,13 @@ void orc_image(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n
 /* SED: Fexc (n_wav) = the sum over grains, in grain order, of every grain's flux. */
 void orc_sed(const orc_optics *o, const orc_view *v, int64_t n_grav, int64_t n_dust, const double *x,
              const double *y, const double *z, const double *N, const int32_t *id, double *Fexc);
+/* Mie efficiencies and phase function of a homogeneous sphere of radius s [um]
+ * at nl wavelengths lam [um] with complex index n - i k (tools/mie_single.c):
+ * Qabs, Qsca (nl) and Qpfunc[nl * t + i] at the nt cosines cost (phase function
+ * normalised to 1 over the sphere).  Returns 0, or -1 if a size parameter
+ * exceeds 7.2e6. */
+int  orc_mie(double s, const double *lam, const double *nre, const double *kim, int nl, const double *cost, int nt,
+             double *Qabs, double *Qsca, double *Qpfunc);
 int  orc_max_threads(void);
 void orc_set_threads(int n);
 

@@ -47,10 +47,19 @@ class OrcView(ctypes.Structure):
 _lib = None
 
 
+REF_ROOT = "/root/reference"
+REF_RUNMIE = os.path.join(HERE, "_ref", "runmie")
+
+
 def build():
     r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    # the reference's own Mie tool, where its sources are present (a checker only)
+    if os.path.isfile(os.path.join(REF_ROOT, "tools", "runmie.c")):
+        r = subprocess.run(["make", "-s", "-C", HERE, "ref", "REF=" + REF_ROOT], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("reference runmie build failed:\n" + r.stdout + r.stderr)
 
 
 def lib():
@@ -84,6 +93,8 @@ def lib():
         L.orc_sed.argtypes = [ctypes.POINTER(OrcOptics), ctypes.POINTER(OrcView), ctypes.c_int64, ctypes.c_int64] + \
             [_D] * 4 + [_I32, _D]
         L.orc_sed.restype = None
+        L.orc_mie.argtypes = [ctypes.c_double, _D, _D, _D, ctypes.c_int, _D, ctypes.c_int, _D, _D, _D]
+        L.orc_mie.restype = ctypes.c_int
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -235,3 +246,15 @@ def sed(t, v, st):
     lib().orc_sed(ctypes.byref(o), ctypes.byref(cv), st.n_grav, st.n_dust, _p(st.x), _p(st.y), _p(st.z), _p(st.N),
                   _p(ids, _I32), _p(out))
     return out
+
+
+def mie(s_um, lam_um, n, k, cost):
+    """Mie tables of one sphere: Qabs, Qsca (n_wav) and the phase function
+    (n_theta, n_wav) at the cosines cost (tools/mie_single.c restated)."""
+    lam, nre, kim, c = (np.ascontiguousarray(a, dtype=np.float64) for a in (lam_um, n, k, cost))
+    nl, nt = len(lam), len(c)
+    qa, qs, qp = np.zeros(nl), np.zeros(nl), np.zeros(nl * nt)
+    rc = lib().orc_mie(float(s_um), _p(lam), _p(nre), _p(kim), nl, _p(c), nt, _p(qa), _p(qs), _p(qp))
+    if rc:
+        raise ValueError("size parameter beyond 7.2e6")
+    return qa, qs, qp.reshape(nt, nl)
