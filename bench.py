@@ -305,6 +305,56 @@ def imaging_run(n_dust=262144, xsize=512, ysize=512, calls=20):
     }
 
 
+def mie_nstop(x):
+    """Terms of the Mie series at size parameter x (the rule of tools/mie_single.c)."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.floor(np.where(x < 0.02, 2.0, x + np.where((x > 8.0) & (x < 4200.0), 4.05, 4.0) * np.cbrt(x) + 2.0))
+
+
+def mie_run(n_sizes=250, amin=0.001, amax=1000.0, n_wav=1000, n_theta=90, cpu_sizes=60):
+    """Mie tables (ddyn_mie): n_sizes radii log-spaced over [amin, amax] um, a
+    smooth synthetic refractive-index table of n_wav wavelengths over 0.09 to
+    1e4 um (the range of the reference's astrosil.lnk), n_theta angles.  Rate:
+    series terms per second (the sum over problems of the terms n_stop of the
+    Mie series, each term with its n_theta angular updates).  CPU baseline: the
+    reference's own runmie (built from its sources into oracle/_ref) on
+    cpu_sizes radii over the same range, same wavelengths and angles, one
+    thread."""
+    import subprocess
+    import tempfile
+    lam = np.geomspace(0.09, 1.0e4, n_wav)
+    nre = 1.5 + 1.9 * np.log10(lam / 0.09) / np.log10(1.0e4 / 0.09)
+    kim = 10.0 ** (-3.0 + 3.0 * np.sin(np.log(lam)) ** 2)
+    lam32, n32, k32 = (a.astype(np.float32).astype(np.float64) for a in (lam, nre, kim))
+    sizes = amin * (amax / amin) ** (np.arange(n_sizes) / (n_sizes - 1.0))
+    cost = np.linspace(-1.0, 1.0, n_theta)
+    terms = lambda sz: float(mie_nstop(2.0 * np.pi * np.asarray(sz)[:, None] / lam32[None, :]).sum())
+    with Engine(0) as eng:
+        eng.mie(sizes[:4], lam32, n32, k32, cost)           # warm-up
+        t0 = time.perf_counter()
+        qa, qs, qp = eng.mie(sizes, lam32, n32, k32, cost)
+        t_gpu = time.perf_counter() - t0
+    out = {"config": {"workload": "Mie tables, synthetic refractive indices", "n_sizes": n_sizes,
+                      "size_um": [amin, amax], "n_wav": n_wav, "wav_um": [0.09, 1.0e4], "n_theta": n_theta,
+                      "max_size_parameter": float(2.0 * np.pi * amax / lam32[0])},
+           "seconds": t_gpu, "terms": terms(sizes), "terms_per_s": terms(sizes) / t_gpu, "cpu_baseline": None}
+    from oracle import oracle as O
+    if os.path.exists(O.REF_RUNMIE):
+        with tempfile.TemporaryDirectory() as d:
+            lnk = os.path.join(d, "syn.lnk")
+            np.savetxt(lnk, np.stack([lam, nre, kim], axis=1), fmt="%.6e", header="lambda n k")
+            t0 = time.perf_counter()
+            subprocess.run([O.REF_RUNMIE, "-Ntheta", str(n_theta), "-Ndust", str(cpu_sizes), "-amin", str(amin),
+                            "-amax", str(amax), "-lnkfile", lnk, "-output", os.path.join(d, "t.dat")],
+                           check=True, capture_output=True)
+            t_cpu = time.perf_counter() - t0
+        cs = amin * (amax / amin) ** (np.arange(cpu_sizes) / (cpu_sizes - 1.0))
+        out["cpu_baseline"] = {"terms_per_s": terms(cs) / t_cpu, "seconds": t_cpu, "cores": 1, "kind": "reference",
+                               "sample": f"tools/runmie built from the reference's sources, {cpu_sizes} radii over "
+                                         "the same range, same wavelengths and angles"}
+    return out
+
+
 def adjoint_run(name, precision=FP32_FORCE):
     """Discrete-adjoint workloads.
 
@@ -504,6 +554,8 @@ def main():
     ap.add_argument("--no-also", action="store_true",
                     help="skip the swarm, large-disk, 1,048,576-grain and imaging lines that the debris-ring run "
                          "adds under 'also'")
+    ap.add_argument("--mie", action="store_true",
+                    help="time only the Mie tables (ddyn_mie) against the reference's runmie on the host")
     ap.add_argument("--imaging", action="store_true",
                     help="time only the images and SEDs of the debris ring's grains (the 'also.imaging' entry)")
     ap.add_argument("--adjoint", action="store_true",
@@ -525,6 +577,9 @@ def main():
         return 2
     if args.imaging:
         print(json.dumps({"imaging": imaging_run()}), flush=True)
+        return 0
+    if args.mie:
+        print(json.dumps({"mie": mie_run()}), flush=True)
         return 0
     if args.adjoint:
         if args.workload not in ("cfg4", "cfg5"):
