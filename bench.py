@@ -20,7 +20,8 @@ baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample
 of whole RK4 steps.  The self-gravitating swarm (65,537 bodies), the large
 disk (1,048,577 bodies) and the debris ring with 1,048,576 grains run too, at
 reduced step counts, under "also", with images and SEDs of the debris ring's
-grains ("also.imaging").
+grains ("also.imaging") and a table of Mie efficiencies and phase functions
+("also.mie").
 
 Prints one JSON line.
 """
@@ -552,8 +553,8 @@ def main():
                     help="steps for the FP64 comparison runs (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-also", action="store_true",
-                    help="skip the swarm, large-disk, 1,048,576-grain and imaging lines that the debris-ring run "
-                         "adds under 'also'")
+                    help="skip the swarm, large-disk, 1,048,576-grain, imaging and Mie lines that the debris-ring "
+                         "run adds under 'also'")
     ap.add_argument("--mie", action="store_true",
                     help="time only the Mie tables (ddyn_mie) against the reference's runmie on the host")
     ap.add_argument("--imaging", action="store_true",
@@ -621,6 +622,7 @@ def main():
             also[key]["fp64_rsq"] = {k: a["fp64_rsq"][k] for k in ("value", "ms_per_step", "steps",
                                                                     "speedup_fp32_vs_fp64_rsq")}
         also["imaging"] = imaging_run()
+        also["mie"] = mie_run()
         out["also"] = also
     print(json.dumps(out), flush=True)
     return 0
