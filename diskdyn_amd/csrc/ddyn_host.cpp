@@ -118,23 +118,36 @@ void copy_str(char* dst, const std::string& s)
 
 namespace ddyn_impl {
 
-// coe2rv (src/functions.cu:6-46), mu = (M + m)(1 - beta).
+// coe2rv (src/functions.cu:6-46), mu = (M + m)(1 - beta).  The sine and
+// cosine of each angle come from one sincos call, as in the reference built
+// by g++ at -O3 (its sin/cos pairs of one argument become sincos calls, and
+// the C library's sincos differs from sin and cos in the last bit for about
+// one argument in 700).
 void coe2rv(double M, double m, double a, double e, double i, double w, double o, double nu,
             double beta, double* r)
 {
+    double snu, cnu, so, co, sw, cw, si, ci;
+    ::sincos(nu, &snu, &cnu);
+    ::sincos(o, &so, &co);
+    ::sincos(w, &sw, &cw);
+    ::sincos(i, &si, &ci);
     const double p = a * (1.0 - e * e);
     const double mu = (M + m) * (1.0 - beta);
-    const double xe = p * std::cos(nu) / (1.0 + e * std::cos(nu));
-    const double ye = p * std::sin(nu) / (1.0 + e * std::cos(nu));
-    const double vxe = -std::sqrt(mu / p) * std::sin(nu);
-    const double vye = std::sqrt(mu / p) * (e + std::cos(nu));
-    const double co = std::cos(o), so = std::sin(o), cw = std::cos(w), sw = std::sin(w);
-    const double ci = std::cos(i), si = std::sin(i);
-    const double r11 = co * cw - so * sw * ci, r12 = -co * sw - so * cw * ci;
-    const double r21 = so * cw + co * sw * ci, r22 = -so * sw + co * cw * ci;
-    const double r31 = sw * si, r32 = cw * si;
-    r[0] = xe * r11 + ye * r12; r[1] = xe * r21 + ye * r22; r[2] = xe * r31 + ye * r32;
-    r[3] = vxe * r11 + vye * r12; r[4] = vxe * r21 + vye * r22; r[5] = vxe * r31 + vye * r32;
+    const double xe = p * cnu / (1.0 + e * cnu);
+    const double ye = p * snu / (1.0 + e * cnu);
+    const double vxe = -std::sqrt(mu / p) * snu;
+    const double vye = std::sqrt(mu / p) * (e + cnu);
+    // The rotation with the reference's products and sums, including the
+    // terms of the zero perifocal z and v_z (they only turn -0 sums into +0).
+    const double ze = 0.0, vze = 0.0;
+    const double r11 = co * cw - so * sw * ci, r12 = -co * sw - so * cw * ci, r13 = so * si;
+    const double r21 = so * cw + co * sw * ci, r22 = -so * sw + co * cw * ci, r23 = -co * si;
+    r[0] = xe * r11 + ye * r12 + ze * r13;
+    r[1] = xe * r21 + ye * r22 + ze * r23;
+    r[2] = xe * sw * si + ye * cw * si + ze * ci;
+    r[3] = vxe * r11 + vye * r12 + vze * r13;
+    r[4] = vxe * r21 + vye * r22 + vze * r23;
+    r[5] = vxe * sw * si + vye * cw * si + vze * ci;
 }
 
 }  // namespace ddyn_impl
@@ -257,9 +270,10 @@ int record(ddyn_config* c, int i, const std::vector<std::string>& d, int& stay)
             if (c->n_planets >= DDYN_MAX_PLANETS) return 44;
             {
                 const int j = c->n_planets++;
-                const double inc = num(d[0]) * M_PI / 180.0, a = num(d[1]), e = num(d[2]);
-                const double nu = num(d[3]) * M_PI / 180.0, o = num(d[4]) * M_PI / 180.0;
-                const double w = num(d[5]) * M_PI / 180.0, m = num(d[6]) * kGrav;
+                // degrees to radians as the reference's i *= M_PI/180.0 (src/read_in.cu:96-100)
+                const double inc = num(d[0]) * (M_PI / 180.0), a = num(d[1]), e = num(d[2]);
+                const double nu = num(d[3]) * (M_PI / 180.0), o = num(d[4]) * (M_PI / 180.0);
+                const double w = num(d[5]) * (M_PI / 180.0), m = num(d[6]) * kGrav;
                 double r[6];
                 coe2rv(c->pm[0], m, a, e, inc, w, o, nu, 0.0, r);
                 c->px[j] = r[0]; c->py[j] = r[1]; c->pz[j] = r[2];

@@ -10,7 +10,7 @@
 // interpolate_stellar_to_grid and calculate_beta (src/functions.cu:243-388),
 // calculate_dust_temp (src/grid.cu:754-792) and getbod, genorb_disk and
 // genorb_blob (src/functions.cu:157-240).  The file formats, units and the
-// formulas are the reference's; the random numbers are not (see Rng).
+// formulas are the reference's, and so is the random stream (Subtractive55).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -288,51 +288,62 @@ double sublimation_radius(const ddyn_config* c, const ddyn_optics* o, const Grid
     return Rmin;
 }
 
-// The seeded random stream of particle generation: Mitchell and Moore's
-// additive lagged-Fibonacci generator X[n] = X[n-24] + X[n-55] mod 2^64
-// (Knuth, The Art of Computer Programming vol. 2, 3.2.2, Algorithm A), its
-// 55 words filled from the seed by SplitMix64 (Steele, Lea and Flood, OOPSLA
-// 2014) with one word forced odd, so the period is at least 2^55 - 1.
-// Uniform deviates are the top 53 bits, offset by half a unit: in (0, 1),
-// never 0 or 1.  The reference draws from Numerical Recipes' ran3 instead, so
-// the particles are the same distributions, not the same numbers.
-class Rng {
+// The random stream of particle generation: Knuth's subtractive generator
+// (The Art of Computer Programming vol. 2, 2nd ed., 3.2.2 and the IN55/IRN55
+// routines of 3.6), X[n] = X[n-55] - X[n-24] mod 10^9, in the parameters the
+// reference draws with (ran3, src/ran3.cu:12-48), so that a .param file gives
+// the reference's particles:
+//  - the 55 starting words: t0 = (161803398 - |seed|) mod 10^9 (C's
+//    truncating remainder), t1 = 1, t[n+1] = t[n-1] - t[n] (+10^9 if
+//    negative), word t[n] in slot 21 n mod 55 (slot 55 for n = 0);
+//  - the recurrence updates one slot per draw, slot i (1..55, cyclic) minus
+//    slot i + 31, which holds X[n-24];
+//  - 220 draws (four passes over the slots) are discarded, and a deviate is
+//    X times the double nearest 10^-9, in [0, 1).
+// The reference seeds with -2 (src/grid.cu:80) and keeps one stream for the
+// whole setup, in the order of the draws below.
+class Subtractive55 {
 public:
-    explicit Rng(int64_t seed)
+    explicit Subtractive55(int64_t seed)
     {
-        uint64_t z = (uint64_t)seed;
-        for (int i = 0; i < 55; ++i) {
-            z += 0x9E3779B97F4A7C15ull;
-            uint64_t x = z;
-            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-            y_[i] = x ^ (x >> 31);
+        const int64_t mag = seed < 0 ? -seed : seed;
+        int64_t prev = (kSeedWord - mag) % kM, cur = 1;
+        slot_[55] = prev;
+        for (int n = 1; n <= 54; ++n) {
+            slot_[(21 * n) % 55] = cur;
+            int64_t nxt = prev - cur;
+            if (nxt < 0) nxt += kM;
+            prev = cur;
+            cur = nxt;
         }
-        y_[0] |= 1ull;
-        j_ = 23; k_ = 54;
-        for (int i = 0; i < 55 * 8; ++i) next();
+        for (int k = 0; k < 4 * 55; ++k) next();
     }
-    uint64_t next()
+    int64_t next()
     {
-        const uint64_t v = y_[k_] + y_[j_];
-        y_[k_] = v;
-        j_ = j_ ? j_ - 1 : 54;
-        k_ = k_ ? k_ - 1 : 54;
+        i_ = i_ == 55 ? 1 : i_ + 1;
+        const int p = i_ + 31 > 55 ? i_ - 24 : i_ + 31;
+        int64_t v = slot_[i_] - slot_[p];
+        if (v < 0) v += kM;
+        slot_[i_] = v;
         return v;
     }
-    double uniform() { return ((double)(next() >> 11) + 0.5) * 0x1.0p-53; }
-    // Box-Muller: sqrt(-2 ln u1) cos(2 pi u2), the normal deviate the reference forms
+    double uniform() { return (double)next() * (1.0 / (double)kM); }
+    // Box-Muller, sqrt(-2 ln u1) cos(2 pi u2), u1 drawn first: the order in
+    // which the reference's host compiler evaluates the two draws of
+    // sqrt(-2.0*log(ran3(&seed)))*cos(PI2*ran3(&seed)) (left operand first).
     double normal()
     {
-        const double u1 = uniform(), u2 = uniform();
+        const double u1 = uniform();
+        const double u2 = uniform();
         return std::sqrt(-2.0 * std::log(u1)) * std::cos(kPI2 * u2);
     }
 
 private:
-    uint64_t y_[55];
-    int j_, k_;
+    static constexpr int64_t kM = 1000000000;
+    static constexpr int64_t kSeedWord = 161803398;
+    int64_t slot_[56] = {};   // slots 1..55
+    int i_ = 0;
 };
-
 struct Sink {
     int64_t n = 0, cap;
     double *x, *y, *z, *vx, *vy, *vz, *N, *mu;
@@ -361,7 +372,7 @@ constexpr int kMaxDraws = 1 << 20;
 // a^(2+gamma) uniform between Rin/(1-e) and Rout/(1+e) (surface density ~
 // r^gamma), i = |normal| atan(dh/dr), node, periapsis and true anomaly uniform.
 // Returns false if no e below 1 was drawn.
-bool orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta, Rng& g, double* r)
+bool orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta, Subtractive55& g, double* r)
 {
     double e = 1.0;
     for (int k = 0; k < kMaxDraws && !(e < 1.0); ++k) e = std::fabs(g.normal()) * D.sigmae;
@@ -385,7 +396,7 @@ bool orbit_disk(const ddyn_config& c, const ddyn_disk& D, double m, double beta,
 // blob's e, i, w, Omega, true anomaly nu_blob + normal sigma / a; for a
 // collision cloud a velocity kick of vkepp % of the orbital speed in an
 // isotropic direction.  Returns false if no positive a was drawn.
-bool orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta, Rng& g, double* r)
+bool orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta, Subtractive55& g, double* r)
 {
     double a = 0.0;
     for (int k = 0; k < kMaxDraws && !(a > 0.0); ++k) a = B.a + g.normal() * B.sigma;
@@ -397,8 +408,10 @@ bool orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta,
         const double ct = -1.0 + 2.0 * g.uniform();
         const double st = std::sqrt(1.0 - ct * ct);
         const double gam = kPI2 * g.uniform();
-        r[3] += voff * st * std::cos(gam);
-        r[4] += voff * st * std::sin(gam);
+        double sg, cg;
+        ::sincos(gam, &sg, &cg);   // one call, as coe2rv's angles
+        r[3] += voff * st * cg;
+        r[4] += voff * st * sg;
         r[5] += voff * ct;
     }
     return true;
@@ -408,17 +421,20 @@ bool orbit_blob(const ddyn_config& c, const ddyn_blob& B, double m, double beta,
 // interval [mmin, mmax] below the current mmax holds one body of
 // dN/dm = C m^-eta (ome = 1 - eta); its mass is the interval's mass.  eta = 1
 // and eta = 2 take the logarithmic forms of the integrals (the reference
-// divides by zero there).  Returns mu = G M of the body.
+// divides by zero there).  Returns mu = G M of the body.  The precisions are
+// the reference's: ome, ome / C, 1 / ome and 1 + ome in double, the powers and
+// the mass in long double, mu rounded to double at the end.
 double next_body(long double& mmax, double C, double eta)
 {
-    const long double ome = 1.0L - (long double)eta;
-    long double mmin, mass;
-    if (eta == 1.0) mmin = mmax * std::exp(-1.0L / (long double)C);
-    else mmin = std::pow(std::pow(mmax, ome) - ome / (long double)C, 1.0L / ome);
-    if (eta == 2.0) mass = (long double)C * std::log(mmax / mmin);
-    else mass = (long double)C * (std::pow(mmax, 1.0L + ome) - std::pow(mmin, 1.0L + ome)) / (1.0L + ome);
+    using LD = long double;
+    const double ome = 1.0 - eta;
+    LD mmin, mass;
+    if (eta == 1.0) mmin = mmax * std::exp(-1.0L / (LD)C);
+    else mmin = std::pow(std::pow(mmax, (LD)ome) - (LD)(ome / C), (LD)(1.0 / ome));
+    if (eta == 2.0) mass = (LD)C * std::log(mmax / mmin);
+    else mass = (LD)C * (std::pow(mmax, (LD)(1.0 + ome)) - std::pow(mmin, (LD)(1.0 + ome))) / (LD)(1.0 + ome);
     mmax = mmin;
-    return (double)(mass / kMSun * kGrav);
+    return (double)(mass / (LD)kMSun * (LD)kGrav);
 }
 
 // The dust mass bins of one disk or blob (src/grid.cu:136-189): its mass
@@ -451,7 +467,7 @@ double bin_number(double C, double eta, double in, double ax)
 
 template <class F>
 int dust_of(const ddyn_config& c, const ddyn_optics* o, const DustBins& b, double C, double eta, const float* beta,
-            int32_t n_bins, Rng& g, Sink& out, F orbit)
+            int32_t n_bins, Subtractive55& g, Sink& out, F orbit)
 {
     const double dmin = o ? o->dmin : 0.0, dmax = o ? o->dmax : std::numeric_limits<double>::infinity();
     for (int l = 0; l < b.ngrid; ++l) {
@@ -756,7 +772,7 @@ int ddyn_generate(const ddyn_config* c, const ddyn_optics* o, const float* beta,
         const double r[6] = {c->px[j], c->py[j], c->pz[j], c->pvx[j], c->pvy[j], c->pvz[j]};
         if (!out.add(r, 1.0, c->pm[j], -1)) return kErrInterp;
     }
-    Rng g(c->seed);
+    Subtractive55 g(c->seed);
     // gravitating bodies of the disks, then of the blobs: the mass ladder from each one's mmax down
     for (int j = 0; j < c->n_disks; ++j) {
         const ddyn_disk& D = c->disks[j];
@@ -782,13 +798,13 @@ int ddyn_generate(const ddyn_config* c, const ddyn_optics* o, const float* beta,
     for (int j = 0; j < c->n_disks; ++j) {
         const ddyn_disk& D = c->disks[j];
         const DustBins b = dust_bins(*c, D.dmmax, D.mmin, D.Nsmall);
-        auto orb = [&](double m, double bt, Rng& gg, double* r) { return orbit_disk(*c, D, m, bt, gg, r); };
+        auto orb = [&](double m, double bt, Subtractive55& gg, double* r) { return orbit_disk(*c, D, m, bt, gg, r); };
         if ((rc = dust_of(*c, o, b, D.C, D.eta, beta, n_bins, g, out, orb))) return rc;
     }
     for (int j = 0; j < c->n_blobs; ++j) {
         const ddyn_blob& B = c->blobs[j];
         const DustBins b = dust_bins(*c, B.dmmax, B.mmin, B.Nsmall);
-        auto orb = [&](double m, double bt, Rng& gg, double* r) { return orbit_blob(*c, B, m, bt, gg, r); };
+        auto orb = [&](double m, double bt, Subtractive55& gg, double* r) { return orbit_blob(*c, B, m, bt, gg, r); };
         if ((rc = dust_of(*c, o, b, B.C, B.eta, beta, n_bins, g, out, orb))) return rc;
     }
     return out.n == n_all ? 0 : kErrInterp;

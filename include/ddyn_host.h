@@ -203,8 +203,11 @@ int ddyn_generate_counts(const ddyn_config *cfg, int64_t *n_mass, int64_t *n_dus
  * speeds as coe2rv does; o gives the optical size range (grains outside it get
  * id -1; may be NULL).  Orbits follow the reference's distributions (surface
  * density ~ r^gamma, |normal| eccentricities and inclinations, uniform angles;
- * Gaussian blobs with an optional collisional velocity kick) from this
- * library's own seeded generator: the same laws, not the reference's numbers.
+ * Gaussian blobs with an optional collisional velocity kick) from the
+ * reference's random stream (Knuth's subtractive generator in the parameters
+ * of src/ran3.cu:12-48, seeded with cfg->seed, -2 as the reference), drawn in
+ * the reference's order: the reference's particles, bit for bit, with the
+ * same C library.
  * Returns 54 when n_all does not match the counts, or when a law's rejection
  * draw (e < 1 for a disk, a > 0 for a blob) fails 2^20 times in a row. */
 int ddyn_generate(const ddyn_config *cfg, const ddyn_optics *o, const float *beta, int32_t n_bins, int64_t n_all,

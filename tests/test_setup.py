@@ -5,10 +5,9 @@ blobs.  What pins them: closed forms (Stefan-Boltzmann luminosity, the
 blackbody equilibrium distance, beta = 0.57 Q_pr (L/M)/(s rho), exact log-log
 interpolation of power laws, the mass ladder's total mass, the bin weights'
 telescoping sum) and the distributions the mathematics fixes (Kolmogorov-
-Smirnov tests of the orbital-element laws).  The reference's own random
-stream (Numerical Recipes' ran3) is not reproduced, so the particles match
-the reference's distributions, not its numbers: parity unpinned at the level
-of individual particles.
+Smirnov tests of the orbital-element laws).  The particles themselves, bit
+for bit against the oracle's restatement of the reference's set-up, are in
+tests/test_setup_stream.py.
 """
 import math
 
@@ -18,6 +17,7 @@ from scipy import stats
 
 from diskdyn_amd import hostio, orbits
 from diskdyn_amd.units import AU, GRAV, MSUN, RSUN
+from oracle import setup_oracle as S
 from tests import opticslib as OL
 
 MEARTH = 5.97e24
@@ -198,7 +198,18 @@ def test_disk_bodies_follow_the_mass_ladder_and_orbit_laws(tmp_path):
     ome = 1 - eta
     mN = (d.mmax ** ome - n * ome / d.C) ** (1 / ome)
     Mtot = d.C * (d.mmax ** (2 - eta) - mN ** (2 - eta)) / (2 - eta)
-    assert abs(m.sum() / Mtot - 1) < 1e-9
+    # The ladder in the reference's precision (ome / C and 1 / ome rounded to
+    # double, the powers in long double) ends 5e-11 off the closed form's mN;
+    # the 30,000 bodies span only a narrow mass range below mmax, so the total
+    # C (mmax^(2-eta) - mN^(2-eta)) / (2-eta) cancels and that becomes 1.1e-7
+    # (6.9e-11 with every step in long double).
+    assert abs(m.sum() / Mtot - 1) < 1e-6
+    # and the masses are the reference's getbod ladder, bit for bit
+    ladder, top = [], S.LD(d.mmax)
+    for _ in range(n):
+        mu, top = S.getbod(top, d.C, 1.0 - d.eta)
+        ladder.append(mu)
+    assert np.array_equal(st.mu[npl:], np.array(ladder))
     a, e, inc, w, o, nu = _elements(st, npl, npl + n, st.mu[0])
     assert stats.kstest(e / 0.1, stats.halfnorm.cdf).pvalue > 1e-3
     assert stats.kstest(inc / math.atan(0.05), stats.halfnorm.cdf).pvalue > 1e-3

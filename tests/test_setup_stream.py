@@ -1,0 +1,193 @@
+"""The reference's particle set-up reproduced bit for bit (CPU):
+ddyn_read_param's planets and ddyn_generate's bodies and grains against the
+oracle's restatement of add_planets / recalc_COM, getbod, genorb_disk,
+genorb_blob and Grid::setup (oracle/setup_oracle.py), on a system with a
+planet, two disks and a cloud and a collision blob, and on the reference's
+own Fomalhaut example.
+
+What pins the oracle: the stream's lag relation Y[n] = Y[n-55] - Y[n-24] mod
+10^9 from each output, the permutation 21 n mod 55 that fills the starting
+words, a second formulation of the stream (a ring of 55 slots, four warm-up
+passes, one slot updated per draw) written here from the same algorithm,
+uniform deviates; getbod's one-body intervals against the closed-form
+integral; coe2rv against vis-viva, the angular momentum and the conic.  No
+output of the reference's generator ships with it and src/ran3.cu includes
+the CUDA header, so the stream is parity unpinned against the reference's own
+numbers: identical here means the reference's algorithm, draw order and
+expressions, with this machine's C library."""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from diskdyn_amd import hostio
+from oracle import oracle as O
+from oracle import setup_oracle as S
+from tests import opticslib as OL
+from tests import reference_example as RE
+
+M = 10 ** 9
+
+
+def slot_ring_words(seed, n):
+    """The stream as 55 slots: fill, four passes in which slot i takes slot
+    i minus slot 1 + (i + 30) mod 55, then per draw the next slot (cyclic)
+    minus the slot 31 further on."""
+    a = [0] * 56
+    mag = abs(seed)
+    prev = S._c_rem(161803398 - mag, M)
+    a[55] = prev
+    cur = 1
+    for i in range(1, 55):
+        k = (21 * i) % 55
+        a[k] = cur
+        nxt = prev - cur
+        if nxt < 0:
+            nxt += M
+        prev, cur = a[k], nxt
+    for _ in range(4):
+        for i in range(1, 56):
+            a[i] -= a[1 + (i + 30) % 55]
+            if a[i] < 0:
+                a[i] += M
+    out, i, p = [], 0, 31
+    for _ in range(n):
+        i = 1 if i == 55 else i + 1
+        p = 1 if p == 55 else p + 1
+        v = a[i] - a[p]
+        if v < 0:
+            v += M
+        a[i] = v
+        out.append(v)
+    return out
+
+
+def test_stream_formulations_agree_and_keep_the_lag_relation():
+    assert sorted((21 * n) % 55 for n in range(1, 55)) == list(range(1, 55))
+    for seed in (-2, 2, -1, -161803397, 12345):
+        r = S.Ran3(seed)
+        words = [r.word() for _ in range(5000)]
+        assert words == slot_ring_words(seed, 5000)
+        w = np.array(words, dtype=np.int64)
+        assert np.all((w[55:] - (w[:-55] - w[31:-24]) % M) == 0)
+        assert w.min() >= 0 and w.max() < M
+    assert [S.Ran3(-2).word() for _ in range(50)] == [S.Ran3(2).word() for _ in range(50)]
+    assert [S.Ran3(-2).word() for _ in range(50)] != [S.Ran3(-3).word() for _ in range(50)]
+    r = S.Ran3(-2)
+    u = np.array([r() for _ in range(200000)])
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    assert np.all((u >= 0) & (u < 1))
+
+
+def test_getbod_intervals_hold_one_body_and_their_mass():
+    """One body per interval: C (mmax^ome - mmin^ome) / ome = 1 (the number
+    integral of dN/dm = C m^-eta), and mu = G C (mmax^(1+ome) - mmin^(1+ome))
+    / (1 + ome) / Msun (the mass integral)."""
+    C, eta = 3.2e22, (3.67 + 2.0) / 3.0
+    ome = 1.0 - eta
+    mmax = S.LD(4.0 * math.pi * 2500.0 / 3.0 * 1e5 ** 3)
+    for _ in range(100):
+        hi = float(mmax)
+        mu, mmax = S.getbod(mmax, C, ome)
+        lo = float(mmax)
+        assert abs(C * (hi ** ome - lo ** ome) / ome - 1.0) < 1e-9
+        mass = C * (hi ** (1 + ome) - lo ** (1 + ome)) / (1 + ome)
+        assert abs(mu / (mass / S.MSUN * S.GRAV) - 1.0) < 1e-9
+
+
+def test_coe2rv_is_a_kepler_orbit():
+    M, m = S.GRAV, 1e-5
+    for a, e, i, w, o, nu, beta in ((30.0, 0.1, 0.2, 1.0, 2.0, 3.0, 0.0), (5.2, 0.6, 1.3, 4.0, 0.3, 5.5, 0.2)):
+        x, y, z, vx, vy, vz = S.coe2rv(M, m, a, e, i, w, o, nu, beta)
+        mu = (M + m) * (1 - beta)
+        r = math.sqrt(x * x + y * y + z * z)
+        assert abs((vx * vx + vy * vy + vz * vz) / (mu * (2 / r - 1 / a)) - 1) < 1e-13
+        h = np.cross([x, y, z], [vx, vy, vz])
+        p = a * (1 - e * e)
+        assert abs(np.linalg.norm(h) / math.sqrt(mu * p) - 1) < 1e-13
+        assert abs(r - p / (1 + e * math.cos(nu))) < 1e-12 * r
+        assert abs(h[2] / np.linalg.norm(h) - math.cos(i)) < 1e-13
+
+
+PLANET = (1.30530, 5.20336301, 0.04839266, 21.62746, 100.55615, 14.75385, 9.54240000e-4)
+DISKS = ("30.0 50.0 0.05 0.1 100.0 1.0 3.67 0.1 -1.0 40 1500\n"
+         "60.0 80.0 0.03 0.02 10.0 3.0 3.5 0.05 0.5 25 700")
+BLOBS = ("40.0 0.1 5.0 10.0 20.0 30.0 100.0 1.0 3.67 0.1 10 300 cloud 2.0\n"
+         "45.0 0.2 3.0 40.0 60.0 90.0 1.0 5.0 3.5 0.01 5 200 collision 3.0")
+SIZES_UM = np.geomspace(0.1, 2000.0, 14)
+WAV_UM = np.geomspace(0.05, 2000.0, 400)
+
+
+def _blob(line):
+    f = line.split()
+    return [float(v) for v in f[:12]] + [f[12], float(f[13])]
+
+
+def _compare(st, ref):
+    for k in ("x", "y", "z", "vx", "vy", "vz", "N", "mu"):
+        got = getattr(st, k)
+        assert got.shape == ref[k].shape, k
+        bad = np.flatnonzero(got != ref[k])
+        assert bad.size == 0, f"{k}: {bad.size} rows differ, first {bad[:5]}: {got[bad[:3]]} vs {ref[k][bad[:3]]}"
+    assert np.array_equal(st.id, ref["id"])
+
+
+def test_planets_and_generation_match_the_reference_setup_bit_for_bit(tmp_path):
+    spec, comp = tmp_path / "star.dat", tmp_path / "bb.dat"
+    OL.write_spectrum(spec, 5800.0)
+    OL.gray_table(comp, SIZES_UM, WAV_UM, qabs=1.0, qsca=0.3, n_theta=21)
+    p = OL.setup_param(tmp_path / "run.param", comp, spec, DISKS, BLOBS)
+    cfg = hostio.read_param(p)
+    P = S.planets(1.0, [PLANET])
+    for j, pl in enumerate(P):
+        got = (cfg.px[j], cfg.py[j], cfg.pz[j], cfg.pvx[j], cfg.pvy[j], cfg.pvz[j], cfg.pm[j])
+        assert got == tuple(pl), j
+    opt = hostio.Optics(cfg.comp.decode(), cfg.specfile.decode(), cfg)
+    hostio.set_dust_grid(cfg, opt)
+    tab = hostio.grain_tables(cfg, opt)
+    info = opt.info()
+    rho = 2.5 * 1000.0
+    disks = [S.disk_record([float(v) for v in line.split()], rho) for line in DISKS.splitlines()]
+    blobs = [S.blob_record(_blob(line), rho) for line in BLOBS.splitlines()]
+    ref = S.setup(P, disks, blobs, rho, info["dmin"], info["dmax"], np.asarray(tab["beta"], np.float32))
+    assert cfg.mmin == S.dust_grid([dict(d) for d in disks], [dict(b) for b in blobs], rho, info["dmin"],
+                                   info["dmax"])[0]
+    for j, d in enumerate(disks):
+        assert cfg.disks[j].C == d["C"] and cfg.disks[j].dmmax == d["dmmax"] and cfg.disks[j].dhdr == d["dhdr"]
+    st = hostio.generate(cfg, opt, tab["beta"])
+    assert st.n_grav == 2 + 40 + 25 + 10 + 5
+    assert np.any(ref["id"] >= 0) and np.any(np.asarray(tab["beta"])[ref["id"][ref["id"] >= 0]] > 0.05)
+    _compare(st, ref)
+
+
+def test_reference_fomalhaut_example_generates_the_reference_particles(tmp_path, monkeypatch):
+    """tests/Fomalhaut.param of the reference (two planets on eccentric orbits,
+    the 132-148 AU disk) with 20,000 grain tracers."""
+    p = RE.setup(str(tmp_path), n_small=20000)
+    lam, n, k = RE.read_lnk(os.path.join(RE.INPUTS, "astrosil.lnk"))
+    sizes = np.geomspace(0.1, 500.0, 6).astype(np.float32).astype(np.float64)
+    cost = np.linspace(-1.0, 1.0, 10).astype(np.float32).astype(np.float64)
+    tabs = [O.mie(s, lam, n, k, cost) for s in sizes]
+    RE.write_composition_from(tmp_path / "optical" / "astrosil.dat", sizes, lam,
+                              np.array([t[0] for t in tabs]), np.array([t[1] for t in tabs]),
+                              np.array([t[2] for t in tabs]), 0.1, 500.0)
+    monkeypatch.chdir(tmp_path)
+    cfg = hostio.read_param(p)
+    text = [ln for ln in open(p).read().splitlines() if ln.strip() and not ln.startswith("#")]
+    star_mass = float(text[6].split()[0])
+    pl = [tuple(float(v) for v in ln.split()) for ln in text[text.index("begin_planets") + 1:text.index("end_planets")]]
+    rho = float(text[text.index("end_planets") + 1].split()[0]) * 1000.0
+    dk = [[float(v) for v in ln.split()] for ln in text[text.index("begin_disks") + 1:text.index("end_disks")]]
+    P = S.planets(star_mass, pl)
+    assert [(cfg.px[j], cfg.pvy[j], cfg.pm[j]) for j in range(3)] == [(q[0], q[4], q[6]) for q in P]
+    opt = hostio.Optics(cfg.comp.decode(), cfg.specfile.decode(), cfg)
+    hostio.set_dust_grid(cfg, opt)
+    tab = hostio.grain_tables(cfg, opt)
+    info = opt.info()
+    ref = S.setup(P, [S.disk_record(d, rho) for d in dk], [], rho, info["dmin"], info["dmax"],
+                  np.asarray(tab["beta"], np.float32))
+    st = hostio.generate(cfg, opt, tab["beta"])
+    assert abs(st.n_dust - 20000) < 200
+    _compare(st, ref)
