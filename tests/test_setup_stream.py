@@ -2,8 +2,9 @@
 ddyn_read_param's planets and ddyn_generate's bodies and grains against the
 oracle's restatement of add_planets / recalc_COM, getbod, genorb_disk,
 genorb_blob and Grid::setup (oracle/setup_oracle.py), on a system with a
-planet, two disks and a cloud and a collision blob, and on the reference's
-own Fomalhaut example.
+planet, two disks and a cloud and a collision blob, on the reference's own
+Fomalhaut example (from the committed fixtures), and on each of the
+reference's 13 example .param files when the reference tree is present.
 
 What pins the oracle: the stream's lag relation Y[n] = Y[n-55] - Y[n-24] mod
 10^9 from each output, the permutation 21 n mod 55 that fills the starting
@@ -190,4 +191,54 @@ def test_reference_fomalhaut_example_generates_the_reference_particles(tmp_path,
                   np.asarray(tab["beta"], np.float32))
     st = hostio.generate(cfg, opt, tab["beta"])
     assert abs(st.n_dust - 20000) < 200
+    _compare(st, ref)
+
+
+REF_TESTS = "/root/reference/tests"
+EXAMPLES = sorted(f for f in os.listdir(REF_TESTS) if f.endswith(".param")) if os.path.isdir(REF_TESTS) else []
+
+
+@pytest.mark.skipif(not EXAMPLES, reason="the reference's example inputs are not present")
+@pytest.mark.parametrize("name", EXAMPLES)
+def test_reference_examples_generate_the_reference_particles(tmp_path, name):
+    """Each of the reference's example .param files (tests/*.param), as
+    written, except: 3,000 grain tracers per disk or blob, a blackbody
+    spectrum of the star's temperature in place of its Kurucz file, and a gray
+    composition table in place of its optical table (neither ships with the
+    reference in this form).  Planets, bodies and grains bit for bit."""
+    lines = open(os.path.join(REF_TESTS, name)).read().splitlines()
+    nc = [k for k, ln in enumerate(lines) if ln.strip() and not ln.lstrip().startswith("#")]
+    rec = [lines[k].split() for k in nc]
+    spec, comp = tmp_path / "star.dat", tmp_path / "gray.dat"
+    OL.write_spectrum(spec, float(rec[6][2]))
+    OL.gray_table(comp, SIZES_UM, WAV_UM, qabs=1.0, qsca=0.3, n_theta=11)
+    lines[nc[9]] = str(spec)
+    words = [r[0] for r in rec]
+    end_pl = words.index("end_planets")
+    lines[nc[end_pl + 1]] = f"{rec[end_pl + 1][0]} {comp} {rec[end_pl + 1][2]}"
+    disks, blobs = [], []
+    for lo, hi, out, n_field in (("begin_disks", "end_disks", disks, 10), ("begin_blobs", "end_blobs", blobs, 11)):
+        for j in range(words.index(lo) + 1, words.index(hi)):
+            f = rec[j]
+            f[n_field] = "3000"
+            lines[nc[j]] = " ".join(f)
+            out.append(f)
+    p = tmp_path / name
+    p.write_text("\n".join(lines) + "\n")
+    cfg = hostio.read_param(p)
+    planets = [tuple(float(v) for v in rec[j][:7]) for j in range(words.index("begin_planets") + 1, end_pl)]
+    P = S.planets(float(rec[6][0]), planets)
+    assert cfg.n_planets == len(P)
+    for j, pl in enumerate(P):
+        assert (cfg.px[j], cfg.py[j], cfg.pz[j], cfg.pvx[j], cfg.pvy[j], cfg.pvz[j], cfg.pm[j]) == tuple(pl), j
+    opt = hostio.Optics(cfg.comp.decode(), cfg.specfile.decode(), cfg)
+    hostio.set_dust_grid(cfg, opt)
+    tab = hostio.grain_tables(cfg, opt)
+    info = opt.info()
+    rho = float(rec[end_pl + 1][0]) * 1000.0
+    ref = S.setup(P, [S.disk_record([float(v) for v in f[:11]], rho) for f in disks],
+                  [S.blob_record([float(v) for v in f[:12]] + [f[12], float(f[13])], rho) for f in blobs],
+                  rho, info["dmin"], info["dmax"], np.asarray(tab["beta"], np.float32))
+    st = hostio.generate(cfg, opt, tab["beta"])
+    assert st.n_dust > 0
     _compare(st, ref)
