@@ -311,7 +311,8 @@ int main(int argc, char** argv)
                                     st.vx.data(), st.vy.data(), st.vz.data(), st.N.data(), st.mu.data(),
                                     st.id.data(), have_tables ? tb.data() : nullptr, have_tables ? cfg.n_grid : 0,
                                     bp);
-        if (!quiet) std::printf("ddyn: wrote %s\n", name);
+        if (r) std::fprintf(stderr, "ddyn: %s: %s\n", name, ddyn_host_error_text(r));
+        else if (!quiet) std::printf("ddyn: wrote %s\n", name);
         return r;
     };
 
