@@ -529,7 +529,13 @@ def measure(name, steps, fp64_steps, warmup, in_order=False, cpu=False, n_dust=N
                      "traffic": measured_traffic(kname, w32.name, w32.state.n_grav, w32.state.n_dust), "launches": nl, "avg_launch_ms": avg_ms,
                      "flop_per_launch": FLOP_PER_INTERACTION * inter_launch,
                      "note": "achieved = 20 FLOP x interactions per launch / mean HIP-event launch time; "
-                             "traffic = calibrated FETCH_SIZE + WRITE_SIZE bytes per launch from profiles/"},
+                             "traffic = calibrated FETCH_SIZE + WRITE_SIZE bytes per launch from profiles/",
+                     "in_timed_schedule": {
+                         "achieved": FLOP_PER_INTERACTION * per_step * steps / t32 / 1e12,
+                         "frac": FLOP_PER_INTERACTION * per_step * steps / t32 / 1e12 / PEAK_FP32_TFLOPS,
+                         "note": "20 FLOP x every interaction of a step / the timed step: all kernels as the "
+                                 "timed schedule runs them (with grains: two grain launches per step that "
+                                 "overlap each other and the massive stages)"}},
         "state_update": state_update(w32, kt32, steps),
         "host_transfer": {"upload_ms": 1e3 * io32[0], "download_ms": 1e3 * io32[1],
                           "value_incl_transfer": per_step * steps / (t32 + io32[0] + io32[1]),

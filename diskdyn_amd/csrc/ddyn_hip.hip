@@ -107,12 +107,25 @@ __device__ inline double pair_q(double r2, double mu)
 // R^2 = |r_j - r_i|^2 + eps^2 for swarm targets (eps = 0 for star/planet targets),
 // and a zero contribution when R^3 == 0 (pair_q; RSQ: the DDYN_FP64_RSQ form).
 
+// The massive-body kernels (k_pairs64, k_pairs32, k_stage) raise their waves'
+// issue priority.  With grains they run beside the grain kernel of the
+// previous step (overlapped schedule), one or two waves per SIMD among seven
+// grain waves: at equal priority they get about an eighth of the issue slots,
+// each of the eight launches of a step takes 40 us instead of 5, and the
+// chain becomes the critical path of the step.  Alone on the GPU (no grains)
+// every wave has the same priority and nothing changes.
+__device__ __forceinline__ void massive_priority()
+{
+    __builtin_amdgcn_s_setprio(3);
+}
+
 template <int TPT, bool RSQ>
 __global__ __launch_bounds__(kWG) void k_pairs64(const double4* __restrict__ pos, int n_grav,
                                                  int n_planets, int slice_len, double eps2,
                                                  double* __restrict__ part)
 {
     __shared__ double4 tile[kTile];
+    massive_priority();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wbase = blockIdx.x * (kWG * TPT) + wave * (64 * TPT);
@@ -232,6 +245,7 @@ __global__ __launch_bounds__(kWG) void k_pairs32(const float4* __restrict__ pos3
                                                  float r64sq, double eps2, double* __restrict__ part)
 {
     __shared__ float4 tile[kTile];
+    massive_priority();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wbase = blockIdx.x * (kWG * TPT) + wave * (64 * TPT);
@@ -334,6 +348,7 @@ __device__ inline void store_pos(double4* p64, float4* p32, int i, double x, dou
 template <int STAGE, bool RSQ>
 __global__ __launch_bounds__(kWG) void k_stage(StageArgs A)
 {
+    massive_priority();
     const int i = blockIdx.x * kWG + threadIdx.x;
     if (i >= A.n_grav) return;
     const double4* P = A.p64[STAGE == 4 ? 0 : STAGE];
