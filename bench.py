@@ -20,8 +20,9 @@ baseline is the FP64 oracle (OpenMP) timed on this host on a bounded sample
 of whole RK4 steps.  The self-gravitating swarm (65,537 bodies), the large
 disk (1,048,577 bodies) and the debris ring with 1,048,576 grains run too, at
 reduced step counts, under "also", with images and SEDs of the debris ring's
-grains ("also.imaging") and a table of Mie efficiencies and phase functions
-("also.mie").
+grains ("also.imaging"), a table of Mie efficiencies and phase functions
+("also.mie") and the reference's own Fomalhaut example at its full size of
+2,000,000 grain tracers ("also.fomalhaut").
 
 Prints one JSON line.
 """
@@ -306,6 +307,81 @@ def imaging_run(n_dust=262144, xsize=512, ysize=512, calls=20):
     }
 
 
+FOMALHAUT_INPUTS = os.path.join(ROOT, "tests", "golden", "reference_inputs")
+
+
+def fomalhaut_run(steps=500, warmup=5, cpu=True):
+    """The reference's own example tests/Fomalhaut.param (merope82/DiskDyn
+    v1.01; copied unchanged with its optical/astrosil.lnk and the Kurucz
+    8,500 K spectrum its "Library" line selects into
+    tests/golden/reference_inputs/) at its full size: the A0 star, its two
+    planets and a 132-148 AU disk of 2,000,000 grain tracers.  The reference's
+    workflow: runmie makes optical/astrosil.dat from the refractive indices
+    (150 radii, 0.1-2000 um, 90 angles), then the host library reads the
+    .param file, the optical data and the spectrum, sets the dust grid and the
+    grain tables and generates the particles (the reference's random stream).
+    Timed: `steps` RK4 steps of the reference's own dt (det_step) on the GPU in
+    plain FP64 (the reference's arithmetic; with no swarm the FP32-force form
+    is the same computation) and in FP64_RSQ, state resident in HBM.  CPU
+    baseline: the FP64 oracle stepping the same state on this host's cores."""
+    import shutil
+    import subprocess
+    import tempfile
+    from dataclasses import replace
+    from diskdyn_amd import hostio
+    tmp = tempfile.mkdtemp(prefix="ddyn_fomalhaut_")
+    cwd = os.getcwd()
+    try:
+        for d in ("optical", "Kurucz", "JWST"):
+            os.makedirs(os.path.join(tmp, d))
+        shutil.copyfile(os.path.join(FOMALHAUT_INPUTS, "Kurucz_8500.dat"), os.path.join(tmp, "Kurucz", "Kurucz_8500.dat"))
+        shutil.copyfile(os.path.join(FOMALHAUT_INPUTS, "Fomalhaut.param"), os.path.join(tmp, "Fomalhaut.param"))
+        t0 = time.perf_counter()
+        subprocess.run([os.path.join(ROOT, "diskdyn_amd", "runmie"), "-Ntheta", "90", "-Ndust", "150", "-amin", "0.1",
+                        "-amax", "2000", "-lnkfile", os.path.join(FOMALHAUT_INPUTS, "astrosil.lnk"), "-output",
+                        os.path.join(tmp, "optical", "astrosil.dat"), "--quiet"], check=True, timeout=600)
+        t_mie = time.perf_counter() - t0
+        os.chdir(tmp)
+        t0 = time.perf_counter()
+        cfg = hostio.read_param("Fomalhaut.param")
+        opt = hostio.Optics(cfg.comp.decode(), cfg.specfile.decode(), cfg)
+        hostio.set_dust_grid(cfg, opt)
+        tab = hostio.grain_tables(cfg, opt)
+        st = hostio.generate(cfg, opt, tab["beta"])
+        t_setup = time.perf_counter() - t0
+        st.beta_t, st.betasw_t, st.q_t = tab["beta"], tab["betasw"], tab["q"]
+        prm = hostio.params_from_config(cfg, Rmin=tab["Rmin"])
+        dt = hostio.det_step(cfg, st)
+        per_step = 4 * (st.n_dust * st.n_grav + st.n_grav * (st.n_grav - 1))
+        out = {"config": {"workload": "the reference's tests/Fomalhaut.param, full size: A0 star, two planets, "
+                                      "132-148 AU disk of grain tracers", "n_grav": st.n_grav,
+                          "n_dust": st.n_dust, "dt_yr": dt, "interactions_per_step": per_step},
+               "runmie_s": t_mie, "host_setup_s": t_setup}
+        for label, form in (("fp64", FP64), ("fp64_rsq", FP64_RSQ)):
+            with Engine(0) as eng:
+                eng.upload(st.copy(), replace(prm, precision=form))
+                eng.step(dt, warmup)
+                eng.synchronize()
+                t0 = time.perf_counter()
+                eng.step(dt, steps)
+                eng.synchronize()
+                tt = time.perf_counter() - t0
+            out[label] = {"value": per_step * steps / tt, "unit": "interactions/s", "steps": steps,
+                          "ms_per_step": 1e3 * tt / steps, "steps_per_s": steps / tt,
+                          "grain_steps_per_s": st.n_dust * steps / tt}
+        if cpu:
+            class _W:   # the pieces of a workload the CPU baseline reads
+                pass
+            w = _W()
+            w.state, w.params, w.dt = st, replace(prm, precision=FP64), dt
+            w.interactions_per_eval = lambda: per_step // 4
+            out["cpu_baseline"] = cpu_baseline(w, target_s=10.0)
+        return out
+    finally:
+        os.chdir(cwd)
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def mie_nstop(x):
     """Terms of the Mie series at size parameter x (the rule of tools/mie_single.c)."""
     x = np.asarray(x, dtype=np.float64)
@@ -563,6 +639,8 @@ def main():
                          "run adds under 'also'")
     ap.add_argument("--mie", action="store_true",
                     help="time only the Mie tables (ddyn_mie) against the reference's runmie on the host")
+    ap.add_argument("--fomalhaut", action="store_true",
+                    help="time only the reference's Fomalhaut example at full size (the 'also.fomalhaut' entry)")
     ap.add_argument("--imaging", action="store_true",
                     help="time only the images and SEDs of the debris ring's grains (the 'also.imaging' entry)")
     ap.add_argument("--adjoint", action="store_true",
@@ -584,6 +662,9 @@ def main():
         return 2
     if args.imaging:
         print(json.dumps({"imaging": imaging_run()}), flush=True)
+        return 0
+    if args.fomalhaut:
+        print(json.dumps({"fomalhaut": fomalhaut_run(cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.mie:
         print(json.dumps({"mie": mie_run()}), flush=True)
@@ -629,6 +710,7 @@ def main():
                                                                     "speedup_fp32_vs_fp64_rsq")}
         also["imaging"] = imaging_run()
         also["mie"] = mie_run()
+        also["fomalhaut"] = fomalhaut_run()
         out["also"] = also
     print(json.dumps(out), flush=True)
     return 0
