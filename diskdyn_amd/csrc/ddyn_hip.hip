@@ -875,10 +875,28 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
 // stage positions into the other buffer.  One barrier per step.  The
 // arithmetic is that of k_stage and k_dust (same expressions, same order), so
 // results equal the one-launch-per-kernel schedule bit for bit.
+//
+// With many grains the launch is throughput-bound instead, and the massive
+// wave, redundant in every workgroup, takes wave slots from the grains: the
+// kernel needs about 160 VGPRs (3 waves per SIMD), so with one grain wave per
+// workgroup a CU holds 6 grain waves and 6 massive waves.  From kSmallWideMin
+// grains a workgroup has kSmallWideGW grain waves behind its massive wave,
+// each wave 64 grains.  Three (workgroups of 4 waves, one per SIMD; 3 per CU,
+// 9 grain waves) took the reference's Fomalhaut example (2,000,000 grains,
+// 3 massive bodies) from 0.0955 to 0.0715 ms per step; two gave 0.0848 and
+// five (6-wave workgroups) 0.1046.  Below about 100,000 grains every
+// workgroup of one grain wave is resident at once, the step is latency-bound,
+// and three grain waves were 2-5% slower (4,096 and 32,768 grains).
 constexpr int kSmallMax = 32;
-constexpr int kSmallGrains = 64;                 // grains per workgroup (wave 1)
-constexpr int kSmallWG = 2 * 64;                 // wave 0 massive, wave 1 grains
 constexpr int kSmallChunk = 1024;   // steps per launch of k_small_steps
+#ifndef DDYN_SMALL_WIDE_GW
+#define DDYN_SMALL_WIDE_GW 3
+#endif
+#ifndef DDYN_SMALL_WIDE_MIN
+#define DDYN_SMALL_WIDE_MIN 131072
+#endif
+constexpr int kSmallWideGW = DDYN_SMALL_WIDE_GW;
+constexpr int64_t kSmallWideMin = DDYN_SMALL_WIDE_MIN;
 
 // Pair sum of the star and planets on massive body i at one stage
 // (src/gpu_functions.cu:157-172, as in k_stage).
@@ -912,12 +930,13 @@ __device__ inline void wave_lds_sync()
 // and up to eight planets), so the pair loops unroll and their independent
 // square-root and division chains overlap (the sums keep their order, so every
 // NP gives the bits of NP = 0); NP = 0 takes it from A.
-template <int NP, bool RSQ>
-__global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps)
+template <int NP, bool RSQ, int GW>
+__global__ __launch_bounds__((1 + GW) * 64) void k_small_steps(DustArgs A, int nsteps)
 {
     __shared__ double4 P[2][4][kSmallMax];
     const int lane = threadIdx.x & 63;
-    const bool mwave = threadIdx.x < 64;
+    const int wave = threadIdx.x >> 6;
+    const bool mwave = wave == 0;
     const int ng = A.n_grav, np = NP > 0 ? NP : A.n_planets;
     const double dt = A.dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
     const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
@@ -964,8 +983,8 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
         }
     };
 
-    // wave 1, lane l: grain blockIdx.x * 64 + l
-    const int g = blockIdx.x * kSmallGrains + lane;
+    // grain wave w >= 1, lane l: grain (blockIdx.x * GW + w - 1) * 64 + l
+    const int g = (blockIdx.x * GW + wave - 1) * 64 + lane;
     const int64_t t = (int64_t)ng + g;
     const bool mine = !mwave && g < A.n_dust;
     bool live = false;
@@ -1068,6 +1087,25 @@ __global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps
         const int n = ng;
         A.mfin[lane] = mx; A.mfin[n + lane] = my; A.mfin[2 * n + lane] = mz;
         A.mfin[3 * n + lane] = mvx; A.mfin[4 * n + lane] = mvy; A.mfin[5 * n + lane] = mvz;
+    }
+}
+
+// One launch of k_small_steps, specialised on the number of star + planets.
+template <bool RSQ, int GW>
+void launch_small(const DustArgs& A, unsigned blocks, hipStream_t st, int n)
+{
+    constexpr int wg = (1 + GW) * 64;
+    switch (A.n_planets) {
+    case 1: k_small_steps<1, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 2: k_small_steps<2, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 3: k_small_steps<3, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 4: k_small_steps<4, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 5: k_small_steps<5, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 6: k_small_steps<6, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 7: k_small_steps<7, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 8: k_small_steps<8, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 9: k_small_steps<9, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    default: k_small_steps<0, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
     }
 }
 
@@ -2333,36 +2371,19 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
         // star + planets only: many steps per launch (bounded launch length)
         DustArgs A = dust_args(c, dt);
         A.mfin = c->stage;   // 9 * n_grav of stage scratch, unused by this path
-        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + kSmallGrains - 1) / kSmallGrains);
+        const bool wide = c->n_dust >= kSmallWideMin;
+        const int64_t per_wg = 64 * (wide ? kSmallWideGW : 1);
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + per_wg - 1) / per_wg);
         for (int64_t k = 0; k < nsteps; k += kSmallChunk) {
             const int n = (int)std::min<int64_t>(kSmallChunk, nsteps - k);
             TimedLaunch tl(c, DDYN_KCLASS_DUST);
-            if (c->prm.precision == DDYN_FP64_RSQ) {
-                switch (A.n_planets) {
-                case 1: k_small_steps<1, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 2: k_small_steps<2, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 3: k_small_steps<3, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 4: k_small_steps<4, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 5: k_small_steps<5, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 6: k_small_steps<6, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 7: k_small_steps<7, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 8: k_small_steps<8, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 9: k_small_steps<9, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                default: k_small_steps<0, true><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                }
+            const bool rsq = c->prm.precision == DDYN_FP64_RSQ;
+            if (wide) {
+                if (rsq) launch_small<true, kSmallWideGW>(A, blocks, c->stream, n);
+                else launch_small<false, kSmallWideGW>(A, blocks, c->stream, n);
             } else {
-                switch (A.n_planets) {
-                case 1: k_small_steps<1, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 2: k_small_steps<2, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 3: k_small_steps<3, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 4: k_small_steps<4, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 5: k_small_steps<5, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 6: k_small_steps<6, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 7: k_small_steps<7, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 8: k_small_steps<8, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                case 9: k_small_steps<9, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                default: k_small_steps<0, false><<<blocks, kSmallWG, 0, c->stream>>>(A, n); break;
-                }
+                if (rsq) launch_small<true, 1>(A, blocks, c->stream, n);
+                else launch_small<false, 1>(A, blocks, c->stream, n);
             }
             DDYN_HIP(c, hipGetLastError());
             k_small_commit<<<1, 64, 0, c->stream>>>(A);
