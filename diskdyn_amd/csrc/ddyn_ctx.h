@@ -35,6 +35,7 @@ struct ddyn_ctx {
     hipEvent_t ev_stage[2] = {nullptr, nullptr}, ev_dust[2] = {nullptr, nullptr};
     hipEvent_t ev_dust2[2] = {nullptr, nullptr};
     double* stage = nullptr;     // 9 * n_grav: vr, Sr, Sa
+    double4* small_stg = nullptr;  // star + planets' stage positions of up to 1,024 steps (split small-system form)
     double* part = nullptr;      // nslice * 3 * n_grav
     int nslice = 0, slice_len = 0, tpt = 1;
     double* acc = nullptr;       // 3 * n_all accel scratch

@@ -876,27 +876,18 @@ __global__ __launch_bounds__(kDustWG) __attribute__((amdgpu_waves_per_eu(kDustWa
 // arithmetic is that of k_stage and k_dust (same expressions, same order), so
 // results equal the one-launch-per-kernel schedule bit for bit.
 //
-// With many grains the launch is throughput-bound instead, and the massive
-// wave, redundant in every workgroup, takes wave slots from the grains: the
-// kernel needs about 160 VGPRs (3 waves per SIMD), so with one grain wave per
-// workgroup a CU holds 6 grain waves and 6 massive waves.  From kSmallWideMin
-// grains a workgroup has kSmallWideGW grain waves behind its massive wave,
-// each wave 64 grains.  Three (workgroups of 4 waves, one per SIMD; 3 per CU,
-// 9 grain waves) took the reference's Fomalhaut example (2,000,000 grains,
-// 3 massive bodies) from 0.0955 to 0.0715 ms per step; two gave 0.0848 and
-// five (6-wave workgroups) 0.1046.  Below about 100,000 grains every
-// workgroup of one grain wave is resident at once, the step is latency-bound,
-// and three grain waves were 2-5% slower (4,096 and 32,768 grains).
+// With many grains the launch is throughput-bound instead. The massive wave,
+// redundant in every workgroup, then takes wave slots from the grains (the
+// kernel needs about 160 VGPRs, 3 waves per SIMD: a CU holds 6 grain waves
+// and 6 massive waves), and every grain wave waits at the per-step barrier.
+// From kSmallSplitMin grains the split form runs instead: k_small_massive
+// integrates the star and planets once per launch and records their stage
+// positions, then k_small_grains advances the grains alone (below).
 constexpr int kSmallMax = 32;
 constexpr int kSmallChunk = 1024;   // steps per launch of k_small_steps
-#ifndef DDYN_SMALL_WIDE_GW
-#define DDYN_SMALL_WIDE_GW 3
-#endif
-#ifndef DDYN_SMALL_WIDE_MIN
-#define DDYN_SMALL_WIDE_MIN 131072
-#endif
-constexpr int kSmallWideGW = DDYN_SMALL_WIDE_GW;
-constexpr int64_t kSmallWideMin = DDYN_SMALL_WIDE_MIN;
+constexpr int kSmallWG = 2 * 64;    // wave 0 massive, wave 1 grains
+constexpr int64_t kSmallSplitMin = 131072;
+constexpr int kSmallGrainWG = 256;  // k_small_grains: four waves of 64 grains
 
 // Pair sum of the star and planets on massive body i at one stage
 // (src/gpu_functions.cu:157-172, as in k_stage).
@@ -926,12 +917,65 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One RK4 step of a grain against the four stage positions Ps of the star and
+// planets (k_dust in the FP64 form).  k_small_steps and k_small_grains both
+// call it, so their results agree bit for bit.  A grain removed during the
+// step (sublimation or an exact hit) sets removed and keeps its state.
+template <bool RSQ>
+__device__ __forceinline__ void small_grain_step(const DustArgs& A, const double4 (*Ps)[kSmallMax], int np,
+                                                 double dt, double dt2, double dt6, const GrainCoef& kg,
+                                                 double& x0, double& y0, double& z0, double& vx0, double& vy0,
+                                                 double& vz0, bool& removed)
+{
+    double px = x0, py = y0, pz = z0, ux = vx0, uy = vy0, uz = vz0;
+    double kx[4], ky[4], kz[4], rx[4], ry[4], rz[4];
+    bool gone = false;
+    for (int s = 0; s < 4 && !gone; ++s) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        gone = star_term<RSQ>(A, Ps[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
+        bool hit = false;
+        grain_fp64<RSQ>(Ps[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
+        gone = gone || hit;
+        kx[s] = ax; ky[s] = ay; kz[s] = az;
+        if (s == 0) {
+            rx[1] = __builtin_fma(ax, dt2, vx0); ry[1] = __builtin_fma(ay, dt2, vy0);
+            rz[1] = __builtin_fma(az, dt2, vz0);
+            px = __builtin_fma(vx0, dt2, x0); py = __builtin_fma(vy0, dt2, y0);
+            pz = __builtin_fma(vz0, dt2, z0);
+            ux = rx[1]; uy = ry[1]; uz = rz[1];
+        } else if (s == 1) {
+            rx[2] = __builtin_fma(ax, dt2, vx0); ry[2] = __builtin_fma(ay, dt2, vy0);
+            rz[2] = __builtin_fma(az, dt2, vz0);
+            px = __builtin_fma(rx[1], dt2, x0); py = __builtin_fma(ry[1], dt2, y0);
+            pz = __builtin_fma(rz[1], dt2, z0);
+            ux = rx[2]; uy = ry[2]; uz = rz[2];
+        } else if (s == 2) {
+            rx[3] = __builtin_fma(ax, dt, vx0); ry[3] = __builtin_fma(ay, dt, vy0);
+            rz[3] = __builtin_fma(az, dt, vz0);
+            px = __builtin_fma(rx[2], dt, x0); py = __builtin_fma(ry[2], dt, y0);
+            pz = __builtin_fma(rz[2], dt, z0);
+            ux = rx[3]; uy = ry[3]; uz = rz[3];
+        }
+    }
+    if (gone) {
+        removed = true;
+    } else {
+        const double nx0 = __builtin_fma(dt6, __builtin_fma(2.0, rx[1] + rx[2], vx0) + rx[3], x0);
+        const double ny0 = __builtin_fma(dt6, __builtin_fma(2.0, ry[1] + ry[2], vy0) + ry[3], y0);
+        const double nz0 = __builtin_fma(dt6, __builtin_fma(2.0, rz[1] + rz[2], vz0) + rz[3], z0);
+        vx0 = __builtin_fma(dt6, __builtin_fma(2.0, kx[1] + kx[2], kx[0]) + kx[3], vx0);
+        vy0 = __builtin_fma(dt6, __builtin_fma(2.0, ky[1] + ky[2], ky[0]) + ky[3], vy0);
+        vz0 = __builtin_fma(dt6, __builtin_fma(2.0, kz[1] + kz[2], kz[0]) + kz[3], vz0);
+        x0 = nx0; y0 = ny0; z0 = nz0;
+    }
+}
+
 // NP > 0 fixes the number of star + planets at compile time (1 to 9, the star
 // and up to eight planets), so the pair loops unroll and their independent
 // square-root and division chains overlap (the sums keep their order, so every
 // NP gives the bits of NP = 0); NP = 0 takes it from A.
-template <int NP, bool RSQ, int GW>
-__global__ __launch_bounds__((1 + GW) * 64) void k_small_steps(DustArgs A, int nsteps)
+template <int NP, bool RSQ>
+__global__ __launch_bounds__(kSmallWG) void k_small_steps(DustArgs A, int nsteps)
 {
     __shared__ double4 P[2][4][kSmallMax];
     const int lane = threadIdx.x & 63;
@@ -983,8 +1027,8 @@ __global__ __launch_bounds__((1 + GW) * 64) void k_small_steps(DustArgs A, int n
         }
     };
 
-    // grain wave w >= 1, lane l: grain (blockIdx.x * GW + w - 1) * 64 + l
-    const int g = (blockIdx.x * GW + wave - 1) * 64 + lane;
+    // wave 1, lane l: grain blockIdx.x * 64 + l
+    const int g = blockIdx.x * 64 + lane;
     const int64_t t = (int64_t)ng + g;
     const bool mine = !mwave && g < A.n_dust;
     bool live = false;
@@ -1026,48 +1070,7 @@ __global__ __launch_bounds__((1 + GW) * 64) void k_small_steps(DustArgs A, int n
             if (step + 1 < nsteps) massive_stages(buf ^ 1);
         } else if (live && !removed) {
             // grains against this step's four stage positions (k_dust in the FP64 form)
-            const double4(*Ps)[kSmallMax] = P[buf];
-            double px = x0, py = y0, pz = z0, ux = vx0, uy = vy0, uz = vz0;
-            double kx[4], ky[4], kz[4], rx[4], ry[4], rz[4];
-            bool gone = false;
-            for (int s = 0; s < 4 && !gone; ++s) {
-                double ax = 0.0, ay = 0.0, az = 0.0;
-                gone = star_term<RSQ>(A, Ps[s][0], px, py, pz, ux, uy, uz, kg, ax, ay, az);
-                bool hit = false;
-                grain_fp64<RSQ>(Ps[s], 1, np, px, py, pz, 0.0, ax, ay, az, hit);
-                gone = gone || hit;
-                kx[s] = ax; ky[s] = ay; kz[s] = az;
-                if (s == 0) {
-                    rx[1] = __builtin_fma(ax, dt2, vx0); ry[1] = __builtin_fma(ay, dt2, vy0);
-                    rz[1] = __builtin_fma(az, dt2, vz0);
-                    px = __builtin_fma(vx0, dt2, x0); py = __builtin_fma(vy0, dt2, y0);
-                    pz = __builtin_fma(vz0, dt2, z0);
-                    ux = rx[1]; uy = ry[1]; uz = rz[1];
-                } else if (s == 1) {
-                    rx[2] = __builtin_fma(ax, dt2, vx0); ry[2] = __builtin_fma(ay, dt2, vy0);
-                    rz[2] = __builtin_fma(az, dt2, vz0);
-                    px = __builtin_fma(rx[1], dt2, x0); py = __builtin_fma(ry[1], dt2, y0);
-                    pz = __builtin_fma(rz[1], dt2, z0);
-                    ux = rx[2]; uy = ry[2]; uz = rz[2];
-                } else if (s == 2) {
-                    rx[3] = __builtin_fma(ax, dt, vx0); ry[3] = __builtin_fma(ay, dt, vy0);
-                    rz[3] = __builtin_fma(az, dt, vz0);
-                    px = __builtin_fma(rx[2], dt, x0); py = __builtin_fma(ry[2], dt, y0);
-                    pz = __builtin_fma(rz[2], dt, z0);
-                    ux = rx[3]; uy = ry[3]; uz = rz[3];
-                }
-            }
-            if (gone) {
-                removed = true;
-            } else {
-                const double nx0 = __builtin_fma(dt6, __builtin_fma(2.0, rx[1] + rx[2], vx0) + rx[3], x0);
-                const double ny0 = __builtin_fma(dt6, __builtin_fma(2.0, ry[1] + ry[2], vy0) + ry[3], y0);
-                const double nz0 = __builtin_fma(dt6, __builtin_fma(2.0, rz[1] + rz[2], vz0) + rz[3], z0);
-                vx0 = __builtin_fma(dt6, __builtin_fma(2.0, kx[1] + kx[2], kx[0]) + kx[3], vx0);
-                vy0 = __builtin_fma(dt6, __builtin_fma(2.0, ky[1] + ky[2], ky[0]) + ky[3], vy0);
-                vz0 = __builtin_fma(dt6, __builtin_fma(2.0, kz[1] + kz[2], kz[0]) + kz[3], vz0);
-                x0 = nx0; y0 = ny0; z0 = nz0;
-            }
+            small_grain_step<RSQ>(A, P[buf], np, dt, dt2, dt6, kg, x0, y0, z0, vx0, vy0, vz0, removed);
         }
         __syncthreads();
     }
@@ -1090,22 +1093,159 @@ __global__ __launch_bounds__((1 + GW) * 64) void k_small_steps(DustArgs A, int n
     }
 }
 
+// The split form, for many grains: the star and planets alone first, then the
+// grains against their recorded stage positions, so that no grain workgroup
+// carries a massive wave or a per-step barrier.  k_small_massive (one wave,
+// one body per lane) runs the massive wave's code of k_small_steps for nsteps
+// steps and writes the four stage positions of every step to stg, and the
+// final state to mfin (k_small_commit moves it into the state);
+// k_small_grains advances every grain through the same steps against them,
+// one grain per lane, reading them with wave-uniform loads.  Same operations
+// in the same order as k_small_steps, so the same bits.
+template <int NP, bool RSQ>
+__global__ __launch_bounds__(64) void k_small_massive(DustArgs A, int nsteps, double4* __restrict__ stg)
+{
+    __shared__ double4 P[4][kSmallMax];
+    const int lane = threadIdx.x;
+    const int ng = A.n_grav, np = NP > 0 ? NP : A.n_planets;
+    const double dt = A.dt, dt6 = dt / 6.0;
+    const double hs[3] = {0.5 * dt, 0.5 * dt, dt};
+    const bool body = lane < ng;
+    double mx = 0, my = 0, mz = 0, mvx = 0, mvy = 0, mvz = 0, mmu = 0;
+    double vrx = 0, vry = 0, vrz = 0, Srx = 0, Sry = 0, Srz = 0, Sax = 0, Say = 0, Saz = 0;
+    if (body) {
+        mx = A.x[lane]; my = A.y[lane]; mz = A.z[lane];
+        mvx = A.vx[lane]; mvy = A.vy[lane]; mvz = A.vz[lane]; mmu = A.mu[lane];
+    }
+    for (int step = 0; step < nsteps; ++step) {
+        double4* out = stg + (int64_t)step * 4 * kSmallMax;
+        // stages 0-2 (k_stage<0..2>)
+        if (body) {
+            P[0][lane] = make_double4(mx, my, mz, mmu);
+            out[lane] = P[0][lane];
+        }
+        wave_lds_sync();
+        for (int s = 0; s < 3; ++s) {
+            if (body) {
+                double ax, ay, az;
+                small_massive_accel<RSQ>(P[s], lane, np, ax, ay, az);
+                const double h = hs[s];
+                const double ux = s ? vrx : mvx, uy = s ? vry : mvy, uz = s ? vrz : mvz;
+                const double rx = __builtin_fma(ax, h, mvx), ry = __builtin_fma(ay, h, mvy),
+                             rz = __builtin_fma(az, h, mvz);
+                P[s + 1][lane] = make_double4(__builtin_fma(ux, h, mx), __builtin_fma(uy, h, my),
+                                              __builtin_fma(uz, h, mz), mmu);
+                out[(s + 1) * kSmallMax + lane] = P[s + 1][lane];
+                if (s == 0) {
+                    Srx = __builtin_fma(2.0, rx, mvx); Sry = __builtin_fma(2.0, ry, mvy);
+                    Srz = __builtin_fma(2.0, rz, mvz);
+                    Sax = ax; Say = ay; Saz = az;
+                } else if (s == 1) {
+                    Srx = __builtin_fma(2.0, rx, Srx); Sry = __builtin_fma(2.0, ry, Sry);
+                    Srz = __builtin_fma(2.0, rz, Srz);
+                    Sax = __builtin_fma(2.0, ax, Sax); Say = __builtin_fma(2.0, ay, Say);
+                    Saz = __builtin_fma(2.0, az, Saz);
+                } else {
+                    Srx = Srx + rx; Sry = Sry + ry; Srz = Srz + rz;
+                    Sax = __builtin_fma(2.0, ax, Sax); Say = __builtin_fma(2.0, ay, Say);
+                    Saz = __builtin_fma(2.0, az, Saz);
+                }
+                vrx = rx; vry = ry; vrz = rz;
+            }
+            wave_lds_sync();
+        }
+        // stage 3 = final combination (k_stage<3>)
+        if (body) {
+            double ax, ay, az;
+            small_massive_accel<RSQ>(P[3], lane, np, ax, ay, az);
+            mx = __builtin_fma(dt6, Srx, mx); my = __builtin_fma(dt6, Sry, my);
+            mz = __builtin_fma(dt6, Srz, mz);
+            mvx = __builtin_fma(dt6, Sax + ax, mvx);
+            mvy = __builtin_fma(dt6, Say + ay, mvy);
+            mvz = __builtin_fma(dt6, Saz + az, mvz);
+        }
+        wave_lds_sync();
+    }
+    if (body) {
+        const int n = ng;
+        A.mfin[lane] = mx; A.mfin[n + lane] = my; A.mfin[2 * n + lane] = mz;
+        A.mfin[3 * n + lane] = mvx; A.mfin[4 * n + lane] = mvy; A.mfin[5 * n + lane] = mvz;
+    }
+}
+
+template <int NP, bool RSQ>
+__global__ __launch_bounds__(kSmallGrainWG) void k_small_grains(DustArgs A, int nsteps,
+                                                               const double4* __restrict__ stg)
+{
+    const int np = NP > 0 ? NP : A.n_planets;
+    const double dt = A.dt, dt2 = dt / 2.0, dt6 = dt / 6.0;
+    const int64_t g = (int64_t)blockIdx.x * kSmallGrainWG + threadIdx.x;
+    if (g >= A.n_dust) return;
+    const int64_t t = (int64_t)A.n_grav + g;
+    if (A.N[t] == 0.0) return;
+    double x0 = A.x[t], y0 = A.y[t], z0 = A.z[t], vx0 = A.vx[t], vy0 = A.vy[t], vz0 = A.vz[t];
+    const double m = A.mu[t];
+    double bet = 0, betsw = 0;
+    const int32_t id = A.id[t];
+    if (A.beta_p) {
+        bet = A.beta_p[g];
+        betsw = A.betasw_p ? A.betasw_p[g] : 0.0;
+    } else if (id >= 0 && id < A.n_bins) {
+        bet = (double)A.beta_t[id];
+        betsw = (double)A.betasw_t[id];
+    }
+    const double qe = (id >= 0 && id < A.n_bins) ? (double)A.q_t[id] : 0.0;
+    const GrainCoef kg = grain_coef(A, bet, betsw, qe, m, A.mu[0]);
+    bool removed = false;
+    for (int step = 0; step < nsteps && !removed; ++step) {
+        const double4(*Ps)[kSmallMax] = reinterpret_cast<const double4(*)[kSmallMax]>(stg + (int64_t)step * 4 * kSmallMax);
+        small_grain_step<RSQ>(A, Ps, np, dt, dt2, dt6, kg, x0, y0, z0, vx0, vy0, vz0, removed);
+    }
+    if (removed) {
+        A.N[t] = 0.0;
+    } else {
+        A.x[t] = x0; A.y[t] = y0; A.z[t] = z0;
+        A.vx[t] = vx0; A.vy[t] = vy0; A.vz[t] = vz0;
+    }
+}
+
 // One launch of k_small_steps, specialised on the number of star + planets.
-template <bool RSQ, int GW>
+template <bool RSQ>
 void launch_small(const DustArgs& A, unsigned blocks, hipStream_t st, int n)
 {
-    constexpr int wg = (1 + GW) * 64;
     switch (A.n_planets) {
-    case 1: k_small_steps<1, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 2: k_small_steps<2, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 3: k_small_steps<3, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 4: k_small_steps<4, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 5: k_small_steps<5, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 6: k_small_steps<6, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 7: k_small_steps<7, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 8: k_small_steps<8, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    case 9: k_small_steps<9, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
-    default: k_small_steps<0, RSQ, GW><<<blocks, wg, 0, st>>>(A, n); break;
+    case 1: k_small_steps<1, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 2: k_small_steps<2, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 3: k_small_steps<3, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 4: k_small_steps<4, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 5: k_small_steps<5, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 6: k_small_steps<6, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 7: k_small_steps<7, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 8: k_small_steps<8, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    case 9: k_small_steps<9, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    default: k_small_steps<0, RSQ><<<blocks, kSmallWG, 0, st>>>(A, n); break;
+    }
+}
+
+// The split form: k_small_massive, then k_small_grains, specialised on the
+// number of star + planets.
+template <bool RSQ>
+void launch_small_split(const DustArgs& A, double4* stg, int64_t n_dust, hipStream_t st, int n)
+{
+    const unsigned gb = (unsigned)((n_dust + kSmallGrainWG - 1) / kSmallGrainWG);
+    switch (A.n_planets) {
+#define DDYN_SPLIT_CASE(K)                                                       \
+    case K:                                                                      \
+        k_small_massive<K, RSQ><<<1, 64, 0, st>>>(A, n, stg);                    \
+        k_small_grains<K, RSQ><<<gb, kSmallGrainWG, 0, st>>>(A, n, stg);          \
+        break;
+    DDYN_SPLIT_CASE(1) DDYN_SPLIT_CASE(2) DDYN_SPLIT_CASE(3) DDYN_SPLIT_CASE(4) DDYN_SPLIT_CASE(5)
+    DDYN_SPLIT_CASE(6) DDYN_SPLIT_CASE(7) DDYN_SPLIT_CASE(8) DDYN_SPLIT_CASE(9)
+#undef DDYN_SPLIT_CASE
+    default:
+        k_small_massive<0, RSQ><<<1, 64, 0, st>>>(A, n, stg);
+        k_small_grains<0, RSQ><<<gb, kSmallGrainWG, 0, st>>>(A, n, stg);
+        break;
     }
 }
 
@@ -1863,7 +2003,7 @@ void free_state(ddyn_ctx* c)
     for (int s = 0; s < 4; ++s) { dfree(c->p64[s]); dfree(c->p32[s]); dfree(c->q64[s]); dfree(c->q32[s]); }
     dfree(c->tape_ck); dfree(c->tape_seg); dfree(c->tape_fin); dfree(c->tape_stg);
     c->have_tape = false;
-    dfree(c->stage); dfree(c->part); dfree(c->acc); dfree(c->rem);
+    dfree(c->stage); dfree(c->part); dfree(c->acc); dfree(c->rem); dfree(c->small_stg);
     dfree(c->beta_p); dfree(c->betasw_p);
     c->use_beta_p = false; c->have_betasw_p = false;
     c->n_grav = c->n_dust = 0;
@@ -2371,19 +2511,20 @@ int ddyn_step(ddyn_ctx* c, double dt, int64_t nsteps)
         // star + planets only: many steps per launch (bounded launch length)
         DustArgs A = dust_args(c, dt);
         A.mfin = c->stage;   // 9 * n_grav of stage scratch, unused by this path
-        const bool wide = c->n_dust >= kSmallWideMin;
-        const int64_t per_wg = 64 * (wide ? kSmallWideGW : 1);
-        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + per_wg - 1) / per_wg);
+        const bool split = c->n_dust >= kSmallSplitMin;
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, (c->n_dust + 63) / 64);
+        if (split && !c->small_stg && (rc = dalloc(c, c->small_stg, (size_t)kSmallChunk * 4 * kSmallMax)))
+            return rc;
         for (int64_t k = 0; k < nsteps; k += kSmallChunk) {
             const int n = (int)std::min<int64_t>(kSmallChunk, nsteps - k);
             TimedLaunch tl(c, DDYN_KCLASS_DUST);
             const bool rsq = c->prm.precision == DDYN_FP64_RSQ;
-            if (wide) {
-                if (rsq) launch_small<true, kSmallWideGW>(A, blocks, c->stream, n);
-                else launch_small<false, kSmallWideGW>(A, blocks, c->stream, n);
+            if (split) {
+                if (rsq) launch_small_split<true>(A, c->small_stg, c->n_dust, c->stream, n);
+                else launch_small_split<false>(A, c->small_stg, c->n_dust, c->stream, n);
             } else {
-                if (rsq) launch_small<true, 1>(A, blocks, c->stream, n);
-                else launch_small<false, 1>(A, blocks, c->stream, n);
+                if (rsq) launch_small<true>(A, blocks, c->stream, n);
+                else launch_small<false>(A, blocks, c->stream, n);
             }
             DDYN_HIP(c, hipGetLastError());
             k_small_commit<<<1, 64, 0, c->stream>>>(A);

@@ -182,13 +182,14 @@ def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps, p
 
 @pytest.mark.parametrize("n_dust", [120_000, 400_000])
 def test_fused_small_steps_more_workgroups_than_fit_at_once(engine_cls, n_dust):
-    """More workgroups of the multi-step kernel than the GPU holds at once, so
-    some start after others have finished: every one must still integrate the
-    massive set from the call's initial state.  120,000 test particles make
-    1,875 workgroups of one grain wave; 400,000 take the wide form (from
-    131,072 grains), 2,084 workgroups of three grain waves.  Single-step calls
-    (the shortest launches) and a longer call, against one launch per kernel,
-    bit for bit, and the planets against the oracle."""
+    """120,000 test particles make 1,875 workgroups of the multi-step kernel,
+    more than the GPU holds at once, so some start after others have finished:
+    every one must still integrate the massive set from the call's initial
+    state.  400,000 take the split form (from 131,072 grains): the star and
+    planets in one wave, then 1,563 workgroups of grains against their
+    recorded stage positions.  Single-step calls (the shortest launches) and a
+    longer call, against one launch per kernel, bit for bit, and the planets
+    against the oracle."""
     w = synth.config1(n_dust=n_dust)
     calls = [1, 1, 1, 7]
     out = []
