@@ -708,9 +708,14 @@ def main():
             also[key]["fp64"] = {k: a["fp64"][k] for k in ("value", "ms_per_step", "steps", "kernel_frac_fp64_peak")}
             also[key]["fp64_rsq"] = {k: a["fp64_rsq"][k] for k in ("value", "ms_per_step", "steps",
                                                                     "speedup_fp32_vs_fp64_rsq")}
-        also["imaging"] = imaging_run()
-        also["mie"] = mie_run()
-        also["fomalhaut"] = fomalhaut_run()
+        # secondary entries: a failure in one is reported in its place and does
+        # not take the headline line with it
+        for key, fn in (("imaging", imaging_run), ("mie", mie_run), ("fomalhaut", fomalhaut_run)):
+            try:
+                also[key] = fn()
+            except Exception as e:   # noqa: BLE001
+                print(f"[bench] also.{key} failed: {e!r}", file=sys.stderr, flush=True)
+                also[key] = {"error": repr(e)}
         out["also"] = also
     print(json.dumps(out), flush=True)
     return 0
