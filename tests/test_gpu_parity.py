@@ -127,11 +127,12 @@ def test_step_fp32(engine_cls, name, nsteps):
     assert np.median(ep) <= 1e-8
 
 
-def _planets_only(n_bodies):
+def _planets_only(n_bodies, n_dust=700):
     """Star, planet and n_bodies - 2 former swarm bodies, all unsoftened
-    planets, with 700 ring grains under radiation pressure and PR drag."""
-    w = synth.config3(n_dust=700, n_swarm=max(n_bodies - 2, 1), precision=FP64)
-    st = subset(w.state, range(n_bodies), range(700))
+    planets, with n_dust ring grains under radiation pressure, PR and
+    stellar-wind drag."""
+    w = synth.config3(n_dust=n_dust, n_swarm=max(n_bodies - 2, 1), precision=FP64)
+    st = subset(w.state, range(n_bodies), range(n_dust))
     return replace(w, state=st, params=replace(w.params, n_planets=n_bodies))
 
 
@@ -144,6 +145,10 @@ FUSED = {
     "bodies6": lambda: _planets_only(6),
     "bodies9": lambda: _planets_only(9),
     "bodies10": lambda: _planets_only(10),
+    # 131,072 grains or more: the split form (k_small_massive, k_small_grains)
+    "ring140k": lambda: synth.config4(n_dust=140_000),
+    "bodies3_140k": lambda: _planets_only(3, 140_000),
+    "bodies10_140k": lambda: _planets_only(10, 140_000),
 }
 
 
@@ -151,11 +156,14 @@ FUSED = {
                                                    ("bodies4", 40, FP64), ("bodies6", 40, FP64), ("bodies9", 40, FP64),
                                                    ("bodies10", 40, FP64), ("cfg1", 300, FP64_RSQ),
                                                    ("cfg4", 50, FP64_RSQ), ("bodies9", 40, FP64_RSQ),
-                                                   ("bodies10", 40, FP64_RSQ)])
+                                                   ("bodies10", 40, FP64_RSQ), ("ring140k", 30, FP64),
+                                                   ("ring140k", 30, FP64_RSQ), ("bodies3_140k", 20, FP64),
+                                                   ("bodies10_140k", 10, FP64_RSQ)])
 def test_fused_small_steps_match_per_kernel_launches(engine_cls, name, nsteps, precision):
     """Star + planets only: the multi-step kernel equals one launch per kernel
     bit for bit, for 1 to 10 massive bodies (the kernel is specialised for 1-9),
-    in the plain FP64 form and in FP64_RSQ."""
+    in the plain FP64 form and in FP64_RSQ; with 140,000 grains in its split
+    form (the star and planets first, then the grains alone)."""
     w = FUSED[name]()
     st = w.state.copy()
     # one grain inside Rmin, one exactly on a planet: removal must agree too
